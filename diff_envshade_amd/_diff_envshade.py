@@ -1,0 +1,394 @@
+"""Drop-in replacement for the reference's `_diff_envshade` extension module.
+
+The ten shader functions keep the names, argument orders, return orders and
+allocation semantics of bindings.cpp:6-19 / gsface_shader.h:3-118
+(zjumsj/diff_envshade); each validates its tensors like the reference's
+TORCH_CHECKs (bruteforce_shader.cu:20-24, raised as RuntimeError), allocates
+its outputs on `normal`'s device and calls the HIP kernels through the C ABI
+(include/diff_envshade.h) on the current stream of that device.
+
+Differences from the reference, all deliberate: launches use the device's
+current stream instead of the legacy default stream; tensor shapes and
+devices are checked (the reference has `// TODO: shape check`); inputs must be
+float32 (the reference admits half/double in CHECK_IS_FLOATING but then reads
+them as float); grad_envmap is summed in a fixed order and is bitwise
+reproducible.
+
+The ten non-shader bindings of the reference (etc.cu: camera rays, env-map
+blur / sampling, nearest mesh points, rotation conversion, bitfield
+extraction) are outside this library and raise NotImplementedError.
+"""
+from __future__ import annotations
+
+import torch
+
+from . import _lib
+
+__all__ = [
+    "bruteforce_diffuse_shader_forward",
+    "bruteforce_diffuse_shader_backward",
+    "bruteforce_specular_shader_forward",
+    "bruteforce_specular_shader_backward",
+    "bruteforce_specular_shader_clamp_forward",
+    "bruteforce_specular_shader_clamp_backward",
+    "bruteforce_specular_shader_forward2",
+    "bruteforce_specular_shader_backward2",
+    "bruteforce_specularvisibility_shader_forward2",
+    "bruteforce_specularvisibility_shader_backward2",
+]
+
+
+# ---- checks mirroring bruteforce_shader.cu:20-24 ---------------------------------
+
+def _check_cuda(x: torch.Tensor, name: str) -> None:
+    if not isinstance(x, torch.Tensor):
+        raise RuntimeError(f"{name} must be a tensor")
+    if not x.is_cuda:
+        raise RuntimeError(f"{name} must be a CUDA tensor")
+    if not x.is_contiguous():
+        raise RuntimeError(f"{name} must be a contiguous tensor")
+
+
+def _check_float(x: torch.Tensor, name: str) -> None:
+    _check_cuda(x, name)
+    if x.dtype != torch.float32:
+        if x.dtype in (torch.float16, torch.float64, torch.bfloat16):
+            raise RuntimeError(f"{name}: expected scalar type Float but found {x.dtype}")
+        raise RuntimeError(f"{name} must be a floating tensor")
+
+
+def _check_int(x: torch.Tensor, name: str) -> None:
+    _check_cuda(x, name)
+    if x.dtype != torch.int32:
+        raise RuntimeError(f"{name} must be an int tensor")
+
+
+def _check_long(x: torch.Tensor, name: str) -> None:
+    _check_cuda(x, name)
+    if x.dtype != torch.int64:
+        raise RuntimeError(f"{name} must be an long tensor")
+
+
+def _check_shape(x: torch.Tensor, name: str, shape: tuple) -> None:
+    if tuple(x.shape) != tuple(shape):
+        # roughness may also be given as a flat (P,) vector
+        if not (len(shape) == 2 and shape[1] == 1 and tuple(x.shape) == (shape[0],)):
+            raise RuntimeError(f"{name} must have shape {tuple(shape)}, got {tuple(x.shape)}")
+
+
+def _same_device(dev: torch.device, tensors) -> None:
+    for name, t in tensors:
+        if t.device != dev:
+            raise RuntimeError(f"{name} is on {t.device} but normal is on {dev}")
+
+
+def _sizes(normal, albedo, envmap):
+    if normal.dim() != 2 or normal.shape[1] != 3:
+        raise RuntimeError(f"normal must have shape (P, 3), got {tuple(normal.shape)}")
+    P = normal.shape[0]
+    if albedo.dim() != 2 or albedo.shape[0] != P:
+        raise RuntimeError(f"albedo must have shape (P, C), got {tuple(albedo.shape)}")
+    C = albedo.shape[1]
+    if C != 3:
+        raise RuntimeError("Channel number not supported!")
+    if envmap.dim() != 3 or envmap.shape[0] != C:
+        raise RuntimeError(f"envmap must have shape (C, H, W), got {tuple(envmap.shape)}")
+    return P, C, int(envmap.shape[1]), int(envmap.shape[2])
+
+
+def _stream(dev: torch.device) -> int:
+    return torch.cuda.current_stream(dev).cuda_stream
+
+
+def _ptr(x):
+    return x.data_ptr() if (x is not None and x.numel() > 0) else None
+
+
+def _workspace(P: int, H: int, W: int, dev: torch.device):
+    n = _lib.workspace_size(P, H, W)
+    if n == 0:
+        return None, 0
+    return torch.empty(n, dtype=torch.uint8, device=dev), n
+
+
+def _material_checks(P, C, normal, view_dir, albedo, specular_albedo, roughness, envmap,
+                     light2obj_rotmat):
+    for name, t in (("normal", normal), ("view_dir", view_dir), ("albedo", albedo),
+                    ("specular_albedo", specular_albedo), ("roughness", roughness),
+                    ("envmap", envmap), ("light2obj_rotmat", light2obj_rotmat)):
+        _check_float(t, name)
+    _check_shape(view_dir, "view_dir", (P, 3))
+    _check_shape(specular_albedo, "specular_albedo", (P, C))
+    _check_shape(roughness, "roughness", (P, 1))
+    _check_shape(light2obj_rotmat, "light2obj_rotmat", (3, 3))
+    _same_device(normal.device, (("view_dir", view_dir), ("albedo", albedo),
+                                 ("specular_albedo", specular_albedo), ("roughness", roughness),
+                                 ("envmap", envmap), ("light2obj_rotmat", light2obj_rotmat)))
+
+
+def _grad_checks(P, C, dev, grads):
+    for name, g in grads:
+        _check_float(g, name)
+        _check_shape(g, name, (P, C))
+        _same_device(dev, ((name, g),))
+
+
+def _vis_checks(P, dev, face_vertex_list, nearest_triangle_id, barycentric_coord, visibility):
+    _check_int(face_vertex_list, "face_vertex_list")
+    _check_long(nearest_triangle_id, "nearest_triangle_id")
+    _check_float(barycentric_coord, "barycentric_coord")
+    _check_int(visibility, "visibility")
+    if face_vertex_list.dim() != 2 or face_vertex_list.shape[0] != 3:
+        raise RuntimeError(f"face_vertex_list must have shape (3, N_face), got "
+                           f"{tuple(face_vertex_list.shape)}")
+    _check_shape(nearest_triangle_id, "nearest_triangle_id", (P,))
+    _check_shape(barycentric_coord, "barycentric_coord", (3, P))
+    if visibility.dim() != 2:
+        raise RuntimeError(f"visibility must have shape (N_slot, N_vertex), got "
+                           f"{tuple(visibility.shape)}")
+    _same_device(dev, (("face_vertex_list", face_vertex_list),
+                       ("nearest_triangle_id", nearest_triangle_id),
+                       ("barycentric_coord", barycentric_coord), ("visibility", visibility)))
+    return (int(face_vertex_list.shape[1]), int(visibility.shape[1]), int(visibility.shape[0]))
+
+
+# ---- diffuse (bindings.cpp:6-7, bruteforce_shader.cu:2687-2792) -------------------
+
+def bruteforce_diffuse_shader_forward(normal, albedo, envmap, light2obj_rotmat):
+    for name, t in (("normal", normal), ("albedo", albedo), ("envmap", envmap),
+                    ("light2obj_rotmat", light2obj_rotmat)):
+        _check_float(t, name)
+    P, C, H, W = _sizes(normal, albedo, envmap)
+    _check_shape(light2obj_rotmat, "light2obj_rotmat", (3, 3))
+    dev = normal.device
+    _same_device(dev, (("albedo", albedo), ("envmap", envmap),
+                       ("light2obj_rotmat", light2obj_rotmat)))
+    shading = torch.empty((P, C), dtype=normal.dtype, device=dev)
+    with torch.cuda.device(dev):
+        _lib.call("des_bruteforce_diffuse_shader_forward", P, C, H, W, _ptr(normal), _ptr(albedo),
+                  _ptr(envmap), _ptr(light2obj_rotmat), _ptr(shading), _stream(dev))
+    return shading
+
+
+def bruteforce_diffuse_shader_backward(grad_shading, normal, albedo, envmap, light2obj_rotmat):
+    for name, t in (("normal", normal), ("albedo", albedo), ("envmap", envmap),
+                    ("light2obj_rotmat", light2obj_rotmat)):
+        _check_float(t, name)
+    P, C, H, W = _sizes(normal, albedo, envmap)
+    _check_shape(light2obj_rotmat, "light2obj_rotmat", (3, 3))
+    dev = normal.device
+    _same_device(dev, (("albedo", albedo), ("envmap", envmap),
+                       ("light2obj_rotmat", light2obj_rotmat)))
+    _grad_checks(P, C, dev, (("grad_shading", grad_shading),))
+    grad_normal = torch.empty((P, 3), dtype=normal.dtype, device=dev)
+    grad_albedo = torch.empty((P, C), dtype=normal.dtype, device=dev)
+    grad_envmap = torch.empty((C, H, W), dtype=normal.dtype, device=dev)
+    with torch.cuda.device(dev):
+        ws, nws = _workspace(P, H, W, dev)
+        _lib.call("des_bruteforce_diffuse_shader_backward", P, C, H, W, _ptr(grad_shading),
+                  _ptr(normal), _ptr(albedo), _ptr(envmap), _ptr(light2obj_rotmat),
+                  _ptr(grad_normal), _ptr(grad_albedo), _ptr(grad_envmap), _ptr(ws), nws,
+                  _stream(dev))
+    return [grad_normal, grad_albedo, grad_envmap]
+
+
+# ---- specular v1 and clamp (bindings.cpp:9-13, bruteforce_shader.cu:2796-3106) -----
+
+def _spec_forward(entry, normal, view_dir, albedo, specular_albedo, roughness, envmap,
+                  light2obj_rotmat, enable_diffuse, enable_specular):
+    _check_float(normal, "normal")
+    _check_float(albedo, "albedo")
+    _check_float(envmap, "envmap")
+    P, C, H, W = _sizes(normal, albedo, envmap)
+    _material_checks(P, C, normal, view_dir, albedo, specular_albedo, roughness, envmap,
+                     light2obj_rotmat)
+    dev = normal.device
+    shading = torch.empty((P, C), dtype=normal.dtype, device=dev)
+    with torch.cuda.device(dev):
+        _lib.call(entry, P, C, H, W, _ptr(normal), _ptr(view_dir), _ptr(albedo),
+                  _ptr(specular_albedo), _ptr(roughness), _ptr(envmap), _ptr(light2obj_rotmat),
+                  int(bool(enable_diffuse)), int(bool(enable_specular)), _ptr(shading),
+                  _stream(dev))
+    return shading
+
+
+def _spec_backward(entry, grad_shading, normal, view_dir, albedo, specular_albedo, roughness,
+                   envmap, light2obj_rotmat, enable_diffuse, enable_specular):
+    _check_float(normal, "normal")
+    _check_float(albedo, "albedo")
+    _check_float(envmap, "envmap")
+    P, C, H, W = _sizes(normal, albedo, envmap)
+    _material_checks(P, C, normal, view_dir, albedo, specular_albedo, roughness, envmap,
+                     light2obj_rotmat)
+    dev = normal.device
+    _grad_checks(P, C, dev, (("grad_shading", grad_shading),))
+    outs = _alloc_grads(P, C, H, W, normal)
+    with torch.cuda.device(dev):
+        ws, nws = _workspace(P, H, W, dev)
+        _lib.call(entry, P, C, H, W, _ptr(grad_shading), _ptr(normal), _ptr(view_dir),
+                  _ptr(albedo), _ptr(specular_albedo), _ptr(roughness), _ptr(envmap),
+                  _ptr(light2obj_rotmat), int(bool(enable_diffuse)), int(bool(enable_specular)),
+                  *(_ptr(o) for o in outs), _ptr(ws), nws, _stream(dev))
+    return outs
+
+
+def _alloc_grads(P, C, H, W, normal):
+    dev, dt = normal.device, normal.dtype
+    return [torch.empty((P, 3), dtype=dt, device=dev),   # grad_normal
+            torch.empty((P, 3), dtype=dt, device=dev),   # grad_view_dir
+            torch.empty((P, C), dtype=dt, device=dev),   # grad_albedo
+            torch.empty((P, C), dtype=dt, device=dev),   # grad_specular_albedo
+            torch.empty((P, 1), dtype=dt, device=dev),   # grad_roughness
+            torch.empty((C, H, W), dtype=dt, device=dev)]  # grad_envmap
+
+
+def bruteforce_specular_shader_forward(normal, view_dir, albedo, specular_albedo, roughness,
+                                       envmap, light2obj_rotmat, enable_diffuse, enable_specular):
+    return _spec_forward("des_bruteforce_specular_shader_forward", normal, view_dir, albedo,
+                         specular_albedo, roughness, envmap, light2obj_rotmat, enable_diffuse,
+                         enable_specular)
+
+
+def bruteforce_specular_shader_backward(grad_shading, normal, view_dir, albedo, specular_albedo,
+                                        roughness, envmap, light2obj_rotmat, enable_diffuse,
+                                        enable_specular):
+    return _spec_backward("des_bruteforce_specular_shader_backward", grad_shading, normal,
+                          view_dir, albedo, specular_albedo, roughness, envmap, light2obj_rotmat,
+                          enable_diffuse, enable_specular)
+
+
+def bruteforce_specular_shader_clamp_forward(normal, view_dir, albedo, specular_albedo, roughness,
+                                             envmap, light2obj_rotmat, enable_diffuse,
+                                             enable_specular):
+    return _spec_forward("des_bruteforce_specular_shader_clamp_forward", normal, view_dir,
+                         albedo, specular_albedo, roughness, envmap, light2obj_rotmat,
+                         enable_diffuse, enable_specular)
+
+
+def bruteforce_specular_shader_clamp_backward(grad_shading, normal, view_dir, albedo,
+                                              specular_albedo, roughness, envmap,
+                                              light2obj_rotmat, enable_diffuse, enable_specular):
+    return _spec_backward("des_bruteforce_specular_shader_clamp_backward", grad_shading, normal,
+                          view_dir, albedo, specular_albedo, roughness, envmap, light2obj_rotmat,
+                          enable_diffuse, enable_specular)
+
+
+# ---- v2 (bindings.cpp:15-16, bruteforce_shader.cu:3108-3259) -----------------------
+
+def bruteforce_specular_shader_forward2(normal, view_dir, albedo, specular_albedo, roughness,
+                                        envmap, light2obj_rotmat, enable_specular):
+    _check_float(normal, "normal")
+    _check_float(albedo, "albedo")
+    _check_float(envmap, "envmap")
+    P, C, H, W = _sizes(normal, albedo, envmap)
+    _material_checks(P, C, normal, view_dir, albedo, specular_albedo, roughness, envmap,
+                     light2obj_rotmat)
+    dev = normal.device
+    shading = torch.empty((P, C), dtype=normal.dtype, device=dev)
+    diffuse_shading = torch.empty((P, C), dtype=normal.dtype, device=dev)
+    with torch.cuda.device(dev):
+        _lib.call("des_bruteforce_specular_shader_forward2", P, C, H, W, _ptr(normal),
+                  _ptr(view_dir), _ptr(albedo), _ptr(specular_albedo), _ptr(roughness),
+                  _ptr(envmap), _ptr(light2obj_rotmat), int(bool(enable_specular)),
+                  _ptr(shading), _ptr(diffuse_shading), _stream(dev))
+    return [shading, diffuse_shading]
+
+
+def bruteforce_specular_shader_backward2(grad_shading, grad_diffuse_shading, normal, view_dir,
+                                         albedo, specular_albedo, roughness, envmap,
+                                         light2obj_rotmat, enable_specular):
+    _check_float(normal, "normal")
+    _check_float(albedo, "albedo")
+    _check_float(envmap, "envmap")
+    P, C, H, W = _sizes(normal, albedo, envmap)
+    _material_checks(P, C, normal, view_dir, albedo, specular_albedo, roughness, envmap,
+                     light2obj_rotmat)
+    dev = normal.device
+    _grad_checks(P, C, dev, (("grad_shading", grad_shading),
+                             ("grad_diffuse_shading", grad_diffuse_shading)))
+    outs = _alloc_grads(P, C, H, W, normal)
+    with torch.cuda.device(dev):
+        ws, nws = _workspace(P, H, W, dev)
+        _lib.call("des_bruteforce_specular_shader_backward2", P, C, H, W, _ptr(grad_shading),
+                  _ptr(grad_diffuse_shading), _ptr(normal), _ptr(view_dir), _ptr(albedo),
+                  _ptr(specular_albedo), _ptr(roughness), _ptr(envmap), _ptr(light2obj_rotmat),
+                  int(bool(enable_specular)), *(_ptr(o) for o in outs), _ptr(ws), nws,
+                  _stream(dev))
+    return outs
+
+
+# ---- v2 + visibility (bindings.cpp:18-19, bruteforce_shader.cu:3261-3453) -----------
+
+def bruteforce_specularvisibility_shader_forward2(normal, view_dir, albedo, specular_albedo,
+                                                  roughness, envmap, light2obj_rotmat,
+                                                  face_vertex_list, nearest_triangle_id,
+                                                  barycentric_coord, visibility,
+                                                  enable_specular):
+    _check_float(normal, "normal")
+    _check_float(albedo, "albedo")
+    _check_float(envmap, "envmap")
+    P, C, H, W = _sizes(normal, albedo, envmap)
+    _material_checks(P, C, normal, view_dir, albedo, specular_albedo, roughness, envmap,
+                     light2obj_rotmat)
+    dev = normal.device
+    NF, NV, NS = _vis_checks(P, dev, face_vertex_list, nearest_triangle_id, barycentric_coord,
+                             visibility)
+    shading = torch.empty((P, C), dtype=normal.dtype, device=dev)
+    diffuse_shading = torch.empty((P, C), dtype=normal.dtype, device=dev)
+    with torch.cuda.device(dev):
+        _lib.call("des_bruteforce_specularvisibility_shader_forward2", P, C, H, W, NF, NV, NS,
+                  _ptr(normal), _ptr(view_dir), _ptr(albedo), _ptr(specular_albedo),
+                  _ptr(roughness), _ptr(envmap), _ptr(light2obj_rotmat), _ptr(face_vertex_list),
+                  _ptr(nearest_triangle_id), _ptr(barycentric_coord), _ptr(visibility),
+                  int(bool(enable_specular)), _ptr(shading), _ptr(diffuse_shading),
+                  _stream(dev))
+    return [shading, diffuse_shading]
+
+
+def bruteforce_specularvisibility_shader_backward2(grad_shading, grad_diffuse_shading, normal,
+                                                   view_dir, albedo, specular_albedo, roughness,
+                                                   envmap, light2obj_rotmat, face_vertex_list,
+                                                   nearest_triangle_id, barycentric_coord,
+                                                   visibility, enable_specular):
+    _check_float(normal, "normal")
+    _check_float(albedo, "albedo")
+    _check_float(envmap, "envmap")
+    P, C, H, W = _sizes(normal, albedo, envmap)
+    _material_checks(P, C, normal, view_dir, albedo, specular_albedo, roughness, envmap,
+                     light2obj_rotmat)
+    dev = normal.device
+    _grad_checks(P, C, dev, (("grad_shading", grad_shading),
+                             ("grad_diffuse_shading", grad_diffuse_shading)))
+    NF, NV, NS = _vis_checks(P, dev, face_vertex_list, nearest_triangle_id, barycentric_coord,
+                             visibility)
+    outs = _alloc_grads(P, C, H, W, normal)
+    with torch.cuda.device(dev):
+        ws, nws = _workspace(P, H, W, dev)
+        _lib.call("des_bruteforce_specularvisibility_shader_backward2", P, C, H, W, NF, NV, NS,
+                  _ptr(grad_shading), _ptr(grad_diffuse_shading), _ptr(normal), _ptr(view_dir),
+                  _ptr(albedo), _ptr(specular_albedo), _ptr(roughness), _ptr(envmap),
+                  _ptr(light2obj_rotmat), _ptr(face_vertex_list), _ptr(nearest_triangle_id),
+                  _ptr(barycentric_coord), _ptr(visibility), int(bool(enable_specular)),
+                  *(_ptr(o) for o in outs), _ptr(ws), nws, _stream(dev))
+    return outs
+
+
+# ---- reference bindings outside this library (bindings.cpp:21-33, etc.cu) ------------
+
+def _not_here(name):
+    def fn(*args, **kwargs):
+        raise NotImplementedError(
+            f"{name} (etc.cu of the reference) is not part of the MI355X shader library")
+    fn.__name__ = name
+    return fn
+
+
+blur_envmap_forward = _not_here("blur_envmap_forward")
+sample_envmap_forward = _not_here("sample_envmap_forward")
+generate_camera_ray_forward = _not_here("generate_camera_ray_forward")
+get_nearest_mesh_points_forward = _not_here("get_nearest_mesh_points_forward")
+get_nearest_mesh_points_backward = _not_here("get_nearest_mesh_points_backward")
+get_uv_of_triangle_forward = _not_here("get_uv_of_triangle_forward")
+get_uv_of_triangle_backward = _not_here("get_uv_of_triangle_backward")
+mt_format_conversion_forward = _not_here("mt_format_conversion_forward")
+extract_bitfield_forward = _not_here("extract_bitfield_forward")

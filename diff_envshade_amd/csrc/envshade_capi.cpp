@@ -1,0 +1,412 @@
+// envshade_capi.cpp -- C ABI of the shader (include/diff_envshade.h): argument
+// validation, launch-shape choice, workspace layout and kernel dispatch.
+// Mirrors the checks and allocation semantics of the reference host wrappers
+// (bruteforce_shader.cu:2687-3453), without allocating.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cstdarg>
+#include <cstdio>
+#include <string>
+
+#include "../../include/diff_envshade.h"
+#include "envshade_internal.h"
+
+namespace {
+
+thread_local std::string g_last_error;
+thread_local hipEvent_t g_bwd_ev_start = nullptr;
+thread_local hipEvent_t g_bwd_ev_stop = nullptr;
+std::atomic<int> g_ntc_fwd_override{0};
+std::atomic<int> g_ntc_bwd_override{0};
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+  return code;
+}
+
+int ok() {
+  g_last_error.clear();
+  return DES_OK;
+}
+
+constexpr int kTileP = 64;
+constexpr int64_t kMaxTexels = int64_t(1) << 24;
+constexpr int kTargetWaves = 8192;  // ~8 waves per SIMD on 256 CUs x 4 SIMDs
+
+int64_t n_tiles(int64_t P) { return (P + kTileP - 1) / kTileP; }
+
+bool is_pow2(int x) { return x > 0 && (x & (x - 1)) == 0; }
+
+// Launch shape as a function of (P, T) only, so results never depend on the
+// device it runs on.
+des::Tiling choose_tiling(int64_t P, int T, bool backward) {
+  const int64_t tiles = n_tiles(P);
+  const int nslot = (T + 31) / 32;
+  const int cap = backward ? 8 : 16;
+  int ntc = 1;
+  const int ov = backward ? g_ntc_bwd_override.load() : g_ntc_fwd_override.load();
+  if (ov > 0) {
+    ntc = std::min(ov, cap);
+  } else {
+    while (ntc * 2 <= cap && ntc * 2 <= nslot && tiles * ntc < kTargetWaves) ntc *= 2;
+  }
+  const int max_slots = des::kMaxSuperChunk / 32;
+  const int slots_per_wave = std::max(1, std::min((nslot + ntc - 1) / ntc, max_slots / ntc));
+  des::Tiling tl;
+  tl.ntc = ntc;
+  tl.cw = slots_per_wave * 32;
+  tl.st = ntc * tl.cw;
+  const size_t table = size_t(32) * tl.st;
+  tl.lds_fwd = std::max(table, size_t(ntc) * des::kForwardAcc * 64 * sizeof(float));
+  tl.lds_bwd = std::max(table, size_t(ntc) * des::kBackwardAcc * 64 * sizeof(float));
+  tl.reduce_rows = des::kReduceRows;
+  return tl;
+}
+
+size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+struct WsLayout {
+  size_t part_off, part2_off, total;
+};
+
+WsLayout ws_layout(int64_t P, int T) {
+  WsLayout w{0, 0, 0};
+  if (P <= 0) return w;
+  const int64_t tiles = n_tiles(P);
+  const int64_t groups = (tiles + des::kReduceRows - 1) / des::kReduceRows;
+  w.part_off = 0;
+  w.part2_off = align256(size_t(tiles) * 3 * T * sizeof(float));
+  w.total = w.part2_off + align256(size_t(groups) * 3 * T * sizeof(double));
+  return w;
+}
+
+int check_common(int64_t P, int32_t C, int32_t H, int32_t W) {
+  if (P < 0) return fail(DES_ERR_INVALID_ARGUMENT, "P must be >= 0 (got %lld)", (long long)P);
+  if (C != 3) return fail(DES_ERR_UNSUPPORTED_CHANNELS, "Channel number not supported!");
+  if (H <= 0 || W <= 0)
+    return fail(DES_ERR_INVALID_ARGUMENT, "envmap must be C x H x W with H, W >= 1 (got %d x %d)", H, W);
+  if (int64_t(H) * W > kMaxTexels)
+    return fail(DES_ERR_INVALID_ARGUMENT, "envmap has %lld texels, limit is %lld",
+                (long long)(int64_t(H) * W), (long long)kMaxTexels);
+  if (n_tiles(P) > int64_t(0x7fffffff))
+    return fail(DES_ERR_INVALID_ARGUMENT, "P = %lld exceeds the grid limit", (long long)P);
+  return DES_OK;
+}
+
+#define REQUIRE(ptr)                                                                     \
+  do {                                                                                   \
+    if ((ptr) == nullptr) return fail(DES_ERR_INVALID_ARGUMENT, #ptr " must not be NULL"); \
+  } while (0)
+
+int check_vis(int64_t P, int T, int32_t N_face, int32_t N_vertex, int32_t N_slot) {
+  if (P == 0) return DES_OK;
+  if (N_face <= 0) return fail(DES_ERR_INVALID_ARGUMENT, "face_vertex_list must be 3 x N_face with N_face >= 1");
+  if (N_vertex <= 0) return fail(DES_ERR_INVALID_ARGUMENT, "visibility must be N_slot x N_vertex with N_vertex >= 1");
+  const int need = (T + 31) / 32;
+  if (N_slot < need)
+    return fail(DES_ERR_INVALID_ARGUMENT,
+                "visibility has %d slots of 32 texels, the %d-texel envmap needs %d", N_slot, T, need);
+  return DES_OK;
+}
+
+int launch_error(hipError_t e, const char* what) {
+  return fail(DES_ERR_LAUNCH, "%s: %s", what, hipGetErrorString(e));
+}
+
+int run_forward(int variant, bool diff, bool spec, des::Problem& pr, hipStream_t stream) {
+  if (pr.P == 0) return ok();
+  const des::Tiling tl = choose_tiling(pr.P, pr.T, false);
+  hipError_t e = des::launch_forward(variant, diff, spec, pr, tl, stream);
+  if (e != hipSuccess) return launch_error(e, "forward launch failed");
+  return ok();
+}
+
+int run_backward(int variant, bool diff, bool spec, des::Problem& pr, void* ws, size_t ws_bytes,
+                 hipStream_t stream) {
+  REQUIRE(pr.gE);
+  if (pr.P == 0) {
+    hipError_t e = des::launch_fill_zero(pr.gE, int64_t(3) * pr.T, stream);
+    if (e != hipSuccess) return launch_error(e, "zero fill failed");
+    return ok();
+  }
+  const WsLayout lay = ws_layout(pr.P, pr.T);
+  if (ws == nullptr || ws_bytes < lay.total)
+    return fail(DES_ERR_WORKSPACE_TOO_SMALL, "workspace of %zu bytes is smaller than the %zu required",
+                ws_bytes, lay.total);
+  if ((reinterpret_cast<uintptr_t>(ws) & 255) != 0)
+    return fail(DES_ERR_INVALID_ARGUMENT, "workspace must be 256-byte aligned");
+  pr.part = reinterpret_cast<float*>(static_cast<char*>(ws) + lay.part_off);
+  pr.part2 = reinterpret_cast<double*>(static_cast<char*>(ws) + lay.part2_off);
+  const des::Tiling tl = choose_tiling(pr.P, pr.T, true);
+  hipEvent_t ev0 = g_bwd_ev_start, ev1 = g_bwd_ev_stop;
+  g_bwd_ev_start = g_bwd_ev_stop = nullptr;
+  if (ev0 && ev1) {
+    hipError_t er = hipEventRecord(ev0, stream);
+    if (er != hipSuccess) return launch_error(er, "timing event record failed");
+  }
+  hipError_t e = des::launch_backward(variant, diff, spec, pr, tl, stream);
+  if (e != hipSuccess) return launch_error(e, "backward launch failed");
+  if (ev0 && ev1) {
+    hipError_t er = hipEventRecord(ev1, stream);
+    if (er != hipSuccess) return launch_error(er, "timing event record failed");
+  }
+  e = des::launch_envgrad_reduce(pr, tl, stream);
+  if (e != hipSuccess) return launch_error(e, "env-map gradient reduce failed");
+  return ok();
+}
+
+des::Problem make_problem(int64_t P, int32_t H, int32_t W) {
+  des::Problem pr{};
+  pr.P = P;
+  pr.H = H;
+  pr.W = W;
+  pr.T = H * W;
+  return pr;
+}
+
+}  // namespace
+
+extern "C" {
+
+DES_API int des_abi_version(void) { return 1; }
+
+DES_API const char* des_last_error(void) { return g_last_error.c_str(); }
+
+DES_API size_t des_backward_workspace_size(int64_t P, int32_t H, int32_t W) {
+  if (P <= 0 || H <= 0 || W <= 0) return 0;
+  return ws_layout(P, H * W).total;
+}
+
+DES_API int des_set_tiling_override(int32_t ntc_forward, int32_t ntc_backward) {
+  if (ntc_forward < 0 || ntc_forward > 16 || (ntc_forward && !is_pow2(ntc_forward)))
+    return fail(DES_ERR_INVALID_ARGUMENT, "ntc_forward must be 0 or a power of two <= 16");
+  if (ntc_backward < 0 || ntc_backward > 8 || (ntc_backward && !is_pow2(ntc_backward)))
+    return fail(DES_ERR_INVALID_ARGUMENT, "ntc_backward must be 0 or a power of two <= 8");
+  g_ntc_fwd_override.store(ntc_forward);
+  g_ntc_bwd_override.store(ntc_backward);
+  return ok();
+}
+
+DES_API int des_set_backward_timing_events(void* start, void* stop) {
+  if ((start == nullptr) != (stop == nullptr))
+    return fail(DES_ERR_INVALID_ARGUMENT, "pass both timing events or neither");
+  g_bwd_ev_start = static_cast<hipEvent_t>(start);
+  g_bwd_ev_stop = static_cast<hipEvent_t>(stop);
+  return ok();
+}
+
+DES_API int des_bruteforce_diffuse_shader_forward(int64_t P, int32_t C, int32_t H, int32_t W,
+                                                  const float* normal, const float* albedo,
+                                                  const float* envmap,
+                                                  const float* light2obj_rotmat, float* shading,
+                                                  void* stream) {
+  if (int rc = check_common(P, C, H, W)) return rc;
+  if (P > 0) {
+    REQUIRE(normal); REQUIRE(albedo); REQUIRE(envmap); REQUIRE(light2obj_rotmat); REQUIRE(shading);
+  }
+  des::Problem pr = make_problem(P, H, W);
+  pr.normal = normal; pr.albedo = albedo; pr.envmap = envmap; pr.rotmat = light2obj_rotmat;
+  pr.sh = shading;
+  return run_forward(des::kDiffuse, true, false, pr, static_cast<hipStream_t>(stream));
+}
+
+DES_API int des_bruteforce_diffuse_shader_backward(
+    int64_t P, int32_t C, int32_t H, int32_t W, const float* grad_shading, const float* normal,
+    const float* albedo, const float* envmap, const float* light2obj_rotmat, float* grad_normal,
+    float* grad_albedo, float* grad_envmap, void* workspace, size_t workspace_bytes,
+    void* stream) {
+  if (int rc = check_common(P, C, H, W)) return rc;
+  if (P > 0) {
+    REQUIRE(grad_shading); REQUIRE(normal); REQUIRE(albedo); REQUIRE(envmap);
+    REQUIRE(light2obj_rotmat); REQUIRE(grad_normal); REQUIRE(grad_albedo);
+  }
+  des::Problem pr = make_problem(P, H, W);
+  pr.g_sh = grad_shading; pr.normal = normal; pr.albedo = albedo; pr.envmap = envmap;
+  pr.rotmat = light2obj_rotmat;
+  pr.gn = grad_normal; pr.ga = grad_albedo; pr.gE = grad_envmap;
+  return run_backward(des::kDiffuse, true, false, pr, workspace, workspace_bytes,
+                      static_cast<hipStream_t>(stream));
+}
+
+static int specular_forward(int variant, int64_t P, int32_t C, int32_t H, int32_t W,
+                            const float* normal, const float* view_dir, const float* albedo,
+                            const float* specular_albedo, const float* roughness,
+                            const float* envmap, const float* light2obj_rotmat, bool diff,
+                            bool spec, float* shading, float* diffuse_shading, void* stream) {
+  if (int rc = check_common(P, C, H, W)) return rc;
+  if (P > 0) {
+    REQUIRE(normal); REQUIRE(envmap); REQUIRE(light2obj_rotmat); REQUIRE(shading);
+    if (diff) REQUIRE(albedo);
+    if (spec) { REQUIRE(view_dir); REQUIRE(specular_albedo); REQUIRE(roughness); }
+    if (variant == des::kClamp) REQUIRE(view_dir);
+    if (variant == des::kV2 || variant == des::kVis) REQUIRE(diffuse_shading);
+  }
+  des::Problem pr = make_problem(P, H, W);
+  pr.normal = normal; pr.view_dir = view_dir; pr.albedo = albedo; pr.spec = specular_albedo;
+  pr.rough = roughness; pr.envmap = envmap; pr.rotmat = light2obj_rotmat;
+  pr.sh = shading; pr.ds = diffuse_shading;
+  return run_forward(variant, diff, spec, pr, static_cast<hipStream_t>(stream));
+}
+
+static int specular_backward(int variant, int64_t P, int32_t C, int32_t H, int32_t W,
+                             const float* grad_shading, const float* grad_diffuse_shading,
+                             const float* normal, const float* view_dir, const float* albedo,
+                             const float* specular_albedo, const float* roughness,
+                             const float* envmap, const float* light2obj_rotmat, bool diff,
+                             bool spec, float* grad_normal, float* grad_view_dir,
+                             float* grad_albedo, float* grad_specular_albedo,
+                             float* grad_roughness, float* grad_envmap, void* workspace,
+                             size_t workspace_bytes, void* stream, des::Problem* vis_in) {
+  if (int rc = check_common(P, C, H, W)) return rc;
+  if (P > 0) {
+    REQUIRE(grad_shading); REQUIRE(normal); REQUIRE(envmap); REQUIRE(light2obj_rotmat);
+    REQUIRE(grad_normal); REQUIRE(grad_view_dir); REQUIRE(grad_albedo);
+    REQUIRE(grad_specular_albedo); REQUIRE(grad_roughness);
+    if (diff) REQUIRE(albedo);
+    if (spec) { REQUIRE(view_dir); REQUIRE(specular_albedo); REQUIRE(roughness); }
+    if (variant == des::kClamp) REQUIRE(view_dir);
+    if (variant == des::kV2 || variant == des::kVis) REQUIRE(grad_diffuse_shading);
+  }
+  des::Problem pr = vis_in ? *vis_in : make_problem(P, H, W);
+  pr.g_sh = grad_shading; pr.g_ds = grad_diffuse_shading;
+  pr.normal = normal; pr.view_dir = view_dir; pr.albedo = albedo; pr.spec = specular_albedo;
+  pr.rough = roughness; pr.envmap = envmap; pr.rotmat = light2obj_rotmat;
+  pr.gn = grad_normal; pr.gv = grad_view_dir; pr.ga = grad_albedo; pr.gs = grad_specular_albedo;
+  pr.gr = grad_roughness; pr.gE = grad_envmap;
+  return run_backward(variant, diff, spec, pr, workspace, workspace_bytes,
+                      static_cast<hipStream_t>(stream));
+}
+
+DES_API int des_bruteforce_specular_shader_forward(
+    int64_t P, int32_t C, int32_t H, int32_t W, const float* normal, const float* view_dir,
+    const float* albedo, const float* specular_albedo, const float* roughness,
+    const float* envmap, const float* light2obj_rotmat, int enable_diffuse, int enable_specular,
+    float* shading, void* stream) {
+  return specular_forward(des::kSpecular, P, C, H, W, normal, view_dir, albedo, specular_albedo,
+                          roughness, envmap, light2obj_rotmat, enable_diffuse != 0,
+                          enable_specular != 0, shading, nullptr, stream);
+}
+
+DES_API int des_bruteforce_specular_shader_backward(
+    int64_t P, int32_t C, int32_t H, int32_t W, const float* grad_shading, const float* normal,
+    const float* view_dir, const float* albedo, const float* specular_albedo,
+    const float* roughness, const float* envmap, const float* light2obj_rotmat,
+    int enable_diffuse, int enable_specular, float* grad_normal, float* grad_view_dir,
+    float* grad_albedo, float* grad_specular_albedo, float* grad_roughness, float* grad_envmap,
+    void* workspace, size_t workspace_bytes, void* stream) {
+  return specular_backward(des::kSpecular, P, C, H, W, grad_shading, nullptr, normal, view_dir,
+                           albedo, specular_albedo, roughness, envmap, light2obj_rotmat,
+                           enable_diffuse != 0, enable_specular != 0, grad_normal, grad_view_dir,
+                           grad_albedo, grad_specular_albedo, grad_roughness, grad_envmap,
+                           workspace, workspace_bytes, stream, nullptr);
+}
+
+DES_API int des_bruteforce_specular_shader_clamp_forward(
+    int64_t P, int32_t C, int32_t H, int32_t W, const float* normal, const float* view_dir,
+    const float* albedo, const float* specular_albedo, const float* roughness,
+    const float* envmap, const float* light2obj_rotmat, int enable_diffuse, int enable_specular,
+    float* shading, void* stream) {
+  return specular_forward(des::kClamp, P, C, H, W, normal, view_dir, albedo, specular_albedo,
+                          roughness, envmap, light2obj_rotmat, enable_diffuse != 0,
+                          enable_specular != 0, shading, nullptr, stream);
+}
+
+DES_API int des_bruteforce_specular_shader_clamp_backward(
+    int64_t P, int32_t C, int32_t H, int32_t W, const float* grad_shading, const float* normal,
+    const float* view_dir, const float* albedo, const float* specular_albedo,
+    const float* roughness, const float* envmap, const float* light2obj_rotmat,
+    int enable_diffuse, int enable_specular, float* grad_normal, float* grad_view_dir,
+    float* grad_albedo, float* grad_specular_albedo, float* grad_roughness, float* grad_envmap,
+    void* workspace, size_t workspace_bytes, void* stream) {
+  return specular_backward(des::kClamp, P, C, H, W, grad_shading, nullptr, normal, view_dir,
+                           albedo, specular_albedo, roughness, envmap, light2obj_rotmat,
+                           enable_diffuse != 0, enable_specular != 0, grad_normal, grad_view_dir,
+                           grad_albedo, grad_specular_albedo, grad_roughness, grad_envmap,
+                           workspace, workspace_bytes, stream, nullptr);
+}
+
+DES_API int des_bruteforce_specular_shader_forward2(
+    int64_t P, int32_t C, int32_t H, int32_t W, const float* normal, const float* view_dir,
+    const float* albedo, const float* specular_albedo, const float* roughness,
+    const float* envmap, const float* light2obj_rotmat, int enable_specular, float* shading,
+    float* diffuse_shading, void* stream) {
+  return specular_forward(des::kV2, P, C, H, W, normal, view_dir, albedo, specular_albedo,
+                          roughness, envmap, light2obj_rotmat, true, enable_specular != 0,
+                          shading, diffuse_shading, stream);
+}
+
+DES_API int des_bruteforce_specular_shader_backward2(
+    int64_t P, int32_t C, int32_t H, int32_t W, const float* grad_shading,
+    const float* grad_diffuse_shading, const float* normal, const float* view_dir,
+    const float* albedo, const float* specular_albedo, const float* roughness,
+    const float* envmap, const float* light2obj_rotmat, int enable_specular, float* grad_normal,
+    float* grad_view_dir, float* grad_albedo, float* grad_specular_albedo, float* grad_roughness,
+    float* grad_envmap, void* workspace, size_t workspace_bytes, void* stream) {
+  return specular_backward(des::kV2, P, C, H, W, grad_shading, grad_diffuse_shading, normal,
+                           view_dir, albedo, specular_albedo, roughness, envmap, light2obj_rotmat,
+                           true, enable_specular != 0, grad_normal, grad_view_dir, grad_albedo,
+                           grad_specular_albedo, grad_roughness, grad_envmap, workspace,
+                           workspace_bytes, stream, nullptr);
+}
+
+DES_API int des_bruteforce_specularvisibility_shader_forward2(
+    int64_t P, int32_t C, int32_t H, int32_t W, int32_t N_face, int32_t N_vertex, int32_t N_slot,
+    const float* normal, const float* view_dir, const float* albedo,
+    const float* specular_albedo, const float* roughness, const float* envmap,
+    const float* light2obj_rotmat, const int32_t* face_vertex_list,
+    const int64_t* nearest_triangle_id, const float* barycentric_coord,
+    const int32_t* visibility, int enable_specular, float* shading, float* diffuse_shading,
+    void* stream) {
+  if (int rc = check_common(P, C, H, W)) return rc;
+  if (int rc = check_vis(P, H * W, N_face, N_vertex, N_slot)) return rc;
+  if (P > 0) {
+    REQUIRE(normal); REQUIRE(albedo); REQUIRE(envmap); REQUIRE(light2obj_rotmat);
+    REQUIRE(face_vertex_list); REQUIRE(nearest_triangle_id); REQUIRE(barycentric_coord);
+    REQUIRE(visibility); REQUIRE(shading); REQUIRE(diffuse_shading);
+    if (enable_specular) { REQUIRE(view_dir); REQUIRE(specular_albedo); REQUIRE(roughness); }
+  }
+  des::Problem pr = make_problem(P, H, W);
+  pr.N_face = N_face; pr.N_vertex = N_vertex;
+  pr.normal = normal; pr.view_dir = view_dir; pr.albedo = albedo; pr.spec = specular_albedo;
+  pr.rough = roughness; pr.envmap = envmap; pr.rotmat = light2obj_rotmat;
+  pr.fvl = face_vertex_list; pr.tri = nearest_triangle_id; pr.bary = barycentric_coord;
+  pr.vis = reinterpret_cast<const uint32_t*>(visibility);
+  pr.sh = shading; pr.ds = diffuse_shading;
+  return run_forward(des::kVis, true, enable_specular != 0, pr, static_cast<hipStream_t>(stream));
+}
+
+DES_API int des_bruteforce_specularvisibility_shader_backward2(
+    int64_t P, int32_t C, int32_t H, int32_t W, int32_t N_face, int32_t N_vertex, int32_t N_slot,
+    const float* grad_shading, const float* grad_diffuse_shading, const float* normal,
+    const float* view_dir, const float* albedo, const float* specular_albedo,
+    const float* roughness, const float* envmap, const float* light2obj_rotmat,
+    const int32_t* face_vertex_list, const int64_t* nearest_triangle_id,
+    const float* barycentric_coord, const int32_t* visibility, int enable_specular,
+    float* grad_normal, float* grad_view_dir, float* grad_albedo, float* grad_specular_albedo,
+    float* grad_roughness, float* grad_envmap, void* workspace, size_t workspace_bytes,
+    void* stream) {
+  if (int rc = check_common(P, C, H, W)) return rc;
+  if (int rc = check_vis(P, H * W, N_face, N_vertex, N_slot)) return rc;
+  if (P > 0) {
+    REQUIRE(face_vertex_list); REQUIRE(nearest_triangle_id); REQUIRE(barycentric_coord);
+    REQUIRE(visibility);
+  }
+  des::Problem pr = make_problem(P, H, W);
+  pr.N_face = N_face; pr.N_vertex = N_vertex;
+  pr.fvl = face_vertex_list; pr.tri = nearest_triangle_id; pr.bary = barycentric_coord;
+  pr.vis = reinterpret_cast<const uint32_t*>(visibility);
+  return specular_backward(des::kVis, P, C, H, W, grad_shading, grad_diffuse_shading, normal,
+                           view_dir, albedo, specular_albedo, roughness, envmap, light2obj_rotmat,
+                           true, enable_specular != 0, grad_normal, grad_view_dir, grad_albedo,
+                           grad_specular_albedo, grad_roughness, grad_envmap, workspace,
+                           workspace_bytes, stream, &pr);
+}
+
+}  // extern "C"

@@ -1,0 +1,70 @@
+// envshade_internal.h -- shared between the C-ABI layer and the kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace des {
+
+// Shader variants (one per pair of reference bindings, bindings.cpp:6-19).
+enum Variant : int {
+  kDiffuse = 0,   // bruteforce_diffuse_shader_*            (bs:231-530)
+  kSpecular = 1,  // bruteforce_specular_shader_*           (bs:532-1038)
+  kClamp = 2,     // bruteforce_specular_shader_clamp_*     (bs:1040-1552)
+  kV2 = 3,        // bruteforce_specular_shader_*2          (bs:1554-2066)
+  kVis = 4,       // bruteforce_specularvisibility_shader_*2 (bs:2068-2683)
+};
+
+// Device pointers and sizes of one shading problem (plain data, passed by value).
+struct Problem {
+  int64_t P;
+  int H, W, T;
+  int N_face, N_vertex;
+  const float* normal;    // P x 3
+  const float* view_dir;  // P x 3
+  const float* albedo;    // P x 3
+  const float* spec;      // P x 3 (specular_albedo)
+  const float* rough;     // P x 1
+  const float* envmap;    // 3 x H x W
+  const float* rotmat;    // 3 x 3 row-major (light2obj_rotmat)
+  const int32_t* fvl;     // 3 x N_face
+  const int64_t* tri;     // P
+  const float* bary;      // 3 x P
+  const uint32_t* vis;    // N_slot x N_vertex
+  const float* g_sh;      // P x 3
+  const float* g_ds;      // P x 3
+  float* sh;              // P x 3
+  float* ds;              // P x 3
+  float* gn;
+  float* gv;
+  float* ga;
+  float* gs;
+  float* gr;
+  float* gE;              // 3 x H x W
+  float* part;            // [ceil(P/64)][3][T] per-tile env-map partials
+  double* part2;          // [ngroups][3][T] stage-1 FP64 sums
+};
+
+// Launch shape: one 64-point tile per workgroup, NTC waves over texel chunks.
+struct Tiling {
+  int ntc;          // waves (texel chunks) per workgroup
+  int cw;           // texels per wave per super-chunk (multiple of 32)
+  int st;           // texels per super-chunk = ntc * cw
+  size_t lds_fwd;   // dynamic LDS bytes, forward
+  size_t lds_bwd;   // dynamic LDS bytes, backward
+  int reduce_rows;  // partial rows summed per stage-1 group
+};
+
+constexpr int kForwardAcc = 6;
+constexpr int kBackwardAcc = 17;
+constexpr int kMaxSuperChunk = 2048;  // texels staged in LDS at once (64 KB)
+constexpr int kReduceRows = 32;
+
+hipError_t launch_forward(int variant, bool diff, bool spec, const Problem& pr, const Tiling& tl,
+                          hipStream_t stream);
+hipError_t launch_backward(int variant, bool diff, bool spec, const Problem& pr, const Tiling& tl,
+                           hipStream_t stream);
+hipError_t launch_envgrad_reduce(const Problem& pr, const Tiling& tl, hipStream_t stream);
+hipError_t launch_fill_zero(float* x, int64_t n, hipStream_t stream);
+
+}  // namespace des

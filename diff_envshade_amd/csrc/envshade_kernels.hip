@@ -1,0 +1,689 @@
+// envshade_kernels.hip -- gfx950 kernels for the brute-force environment-light
+// shader (forward, fused backward, deterministic env-map-gradient reduce).
+//
+// Work decomposition (one launch per op):
+//   * a workgroup owns one tile of 64 points (lane = point) and NTC waves;
+//     wave w shades those 64 points against texel chunk w (CW texels, a whole
+//     number of 32-texel visibility slots) of the current super-chunk;
+//   * the per-texel table (rotated light direction, |l|^2, E_c * dOmega,
+//     dOmega) is computed in FP64 from closed forms and rounded once to fp32
+//     into LDS; every lane reads the same entry (LDS broadcast);
+//   * per-point sums are reduced over the NTC waves through LDS in fixed
+//     order; per-texel env-map-gradient sums are reduced over the 64 lanes
+//     with permlane32/permlane16 swaps + row DPP (4 texels per reduction) and
+//     stored as per-tile partials; two small kernels then sum the partials in
+//     FP64 in a fixed order (bitwise reproducible, no atomics).
+//
+// Semantics follow zjumsj/diff_envshade bruteforce_shader.cu ("bs"):
+//   texel geometry bs:124-147 with the LEFT_TOP_AS_ORIGIN row flip (config.h:11);
+//   Lambert + GGX (F bs:149-159, G bs:161-201, D bs:203-227);
+//   variants: diffuse bs:231-530, specular bs:532-1038, clamp bs:1040-1552,
+//   v2 bs:1554-2066, visibility v2 bs:2068-2683.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "envshade_internal.h"
+
+namespace des {
+
+namespace {
+
+constexpr float kF0 = 0.04f;      // Schlick R0 (bs:30)
+constexpr float kThres = 1e-9f;   // NDF denominator floor (bs:31)
+constexpr float kInvPi = 0.318309886183790671538f;
+constexpr float kPi = 3.14159265358979323846f;
+constexpr double kPiD = 3.14159265358979323846;
+
+__device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float rsq(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ __forceinline__ float sat(float x) { return __builtin_amdgcn_fmed3f(x, 0.f, 1.f); }
+
+// ---- cross-lane helpers (wave64) -------------------------------------------------
+
+template <int CTRL>
+__device__ __forceinline__ float dpp(float x) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), CTRL, 0xf, 0xf, false));
+}
+
+// Sum of the 16 lanes of each row, returned in every lane of that row.
+__device__ __forceinline__ float row_allreduce(float s) {
+  s += dpp<0x128>(s);  // row_ror:8
+  s += dpp<0x124>(s);  // row_ror:4
+  s += dpp<0x122>(s);  // row_ror:2
+  s += dpp<0x121>(s);  // row_ror:1
+  return s;
+}
+
+// Reduce four per-lane values q0..q3 (one per texel) over all 64 lanes.
+// Result: every lane of row k (lanes 16k..16k+15) holds the total of texel
+// kRowTexel[k] = {0, 2, 1, 3}[k].
+__device__ __forceinline__ float reduce4(float q0, float q1, float q2, float q3) {
+  // upper 32 lanes of the first operand <-> lower 32 lanes of the second
+  auto a = __builtin_amdgcn_permlane32_swap(__float_as_uint(q0), __float_as_uint(q1), false, false);
+  auto b = __builtin_amdgcn_permlane32_swap(__float_as_uint(q2), __float_as_uint(q3), false, false);
+  float s01 = __uint_as_float(a[0]) + __uint_as_float(a[1]);  // rows 0,1: q0   rows 2,3: q1
+  float s23 = __uint_as_float(b[0]) + __uint_as_float(b[1]);  // rows 0,1: q2   rows 2,3: q3
+  // odd rows of the first operand <-> even rows of the second
+  auto c = __builtin_amdgcn_permlane16_swap(__float_as_uint(s01), __float_as_uint(s23), false, false);
+  float s = __uint_as_float(c[0]) + __uint_as_float(c[1]);  // row0 q0, row1 q2, row2 q1, row3 q3
+  return row_allreduce(s);
+}
+
+// ---- per-texel table ---------------------------------------------------------
+
+// Texels [t0, t0+n) of the lat-long map into LDS slots [0, n); slots [n, n_alloc)
+// are zero so padded texels contribute exactly nothing.
+//   storage texel t: row r = t / W, column x = t % W, flipped row iy = H-1-r;
+//   theta = ((iy+.5)/H - .5) pi, phi = 2 pi (x+.5)/W, d = (cos th cos ph, sin th, cos th sin ph)
+//   dOmega = (sin(pi(iy+1)/H - pi/2) - sin(pi iy/H - pi/2)) 2pi/W
+//          = 2 sin(pi/(2H)) cos(theta) 2pi/W   (exact identity, no cancellation)
+//   l = M d with M row-major.
+__device__ void build_table(const Problem& pr, int t0, int n, int n_alloc, float4* geo,
+                            float4* rad) {
+  const int H = pr.H, W = pr.W, T = pr.T;
+  double M[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) M[i] = (double)pr.rotmat[i];
+  const double kdw = 4.0 * kPiD * sinpi(0.5 / (double)H) / (double)W;
+  for (int i = threadIdx.x; i < n_alloc; i += blockDim.x) {
+    float4 g = make_float4(0.f, 0.f, 0.f, 0.f);
+    float4 e = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (i < n) {
+      const int t = t0 + i;
+      const int r = t / W;
+      const int x = t - r * W;
+      const int iy = H - 1 - r;
+      double sth, cth, sph, cph;
+      sincospi(((double)iy + 0.5) / (double)H - 0.5, &sth, &cth);
+      sincospi((2.0 * (double)x + 1.0) / (double)W, &sph, &cph);
+      const double dx = cth * cph, dy = sth, dz = cth * sph;
+      const float lx = (float)(M[0] * dx + M[1] * dy + M[2] * dz);
+      const float ly = (float)(M[3] * dx + M[4] * dy + M[5] * dz);
+      const float lz = (float)(M[6] * dx + M[7] * dy + M[8] * dz);
+      const float ll = (float)((double)lx * lx + (double)ly * ly + (double)lz * lz);
+      const double dw = cth * kdw;
+      g = make_float4(lx, ly, lz, ll);
+      e = make_float4((float)(pr.envmap[t] * dw), (float)(pr.envmap[T + t] * dw),
+                      (float)(pr.envmap[2 * T + t] * dw), (float)dw);
+    }
+    geo[i] = g;
+    rad[i] = e;
+  }
+}
+
+// ---- per-point state ---------------------------------------------------------
+
+template <int V, bool SPEC>
+struct PointState {
+  float nx, ny, nz;
+  float vx, vy, vz;
+  float vv, nv;        // |v|^2, n.v (unclamped)
+  float k, omk, ib;    // Smith-Schlick k, 1-k, 1/(max(n.v,0)(1-k)+k)
+  float a2, am1;       // alpha^2 = r^4, alpha^2 - 1
+  float live;          // 0 for padding lanes and culled points (clamp variant)
+  // visibility
+  int vert0, vert1, vert2;
+  float c0, c1, c2;
+};
+
+template <int V, bool SPEC>
+__device__ __forceinline__ void load_point(const Problem& pr, int64_t p, bool valid,
+                                           PointState<V, SPEC>& s) {
+  s.nx = s.ny = s.nz = 0.f;
+  s.vx = s.vy = s.vz = 0.f;
+  float r = 0.f;
+  if (valid) {
+    s.nx = pr.normal[3 * p + 0];
+    s.ny = pr.normal[3 * p + 1];
+    s.nz = pr.normal[3 * p + 2];
+    if (SPEC || V == kClamp) {
+      s.vx = pr.view_dir[3 * p + 0];
+      s.vy = pr.view_dir[3 * p + 1];
+      s.vz = pr.view_dir[3 * p + 2];
+    }
+    if (SPEC) r = pr.rough[p];
+  }
+  s.vv = s.vx * s.vx + s.vy * s.vy + s.vz * s.vz;
+  s.nv = s.nx * s.vx + s.ny * s.vy + s.nz * s.vz;
+  // bs:2126-2129: alpha2 = r^4, k = (r+1)^2 / 8
+  const float r2 = r * r;
+  s.a2 = r2 * r2;
+  s.am1 = s.a2 - 1.f;
+  s.k = (r + 1.f) * (r + 1.f) * 0.125f;
+  s.omk = 1.f - s.k;
+  s.ib = 1.f / (fmaxf(s.nv, 0.f) * s.omk + s.k);
+  s.live = valid ? 1.f : 0.f;
+  if (V == kClamp && !(s.nv > 0.f)) s.live = 0.f;  // per-point back-face cull (bs:1111)
+  s.vert0 = s.vert1 = s.vert2 = 0;
+  s.c0 = s.c1 = s.c2 = 0.f;
+  if (V == kVis && valid) {
+    // bs:2145-2151: nearest triangle -> 3 vertices, barycentric weights (3 x P)
+    const int64_t f = pr.tri[p];
+    if (f >= 0 && f < (int64_t)pr.N_face) {
+      int a = pr.fvl[f], b = pr.fvl[(int64_t)pr.N_face + f], c = pr.fvl[2 * (int64_t)pr.N_face + f];
+      // out-of-range ids read as occluded instead of faulting
+      const bool ok = a >= 0 && a < pr.N_vertex && b >= 0 && b < pr.N_vertex && c >= 0 &&
+                      c < pr.N_vertex;
+      if (ok) {
+        s.vert0 = a;
+        s.vert1 = b;
+        s.vert2 = c;
+        s.c0 = pr.bary[p];
+        s.c1 = pr.bary[pr.P + p];
+        s.c2 = pr.bary[2 * pr.P + p];
+      }
+    }
+  }
+}
+
+// Visibility words of one 32-texel slot (bs:2180-2185): word for vertex v of
+// slot s at visibility[s * N_vertex + v].
+struct VisWords {
+  uint32_t w0, w1, w2;
+};
+
+template <int V, bool SPEC>
+__device__ __forceinline__ VisWords load_slot(const Problem& pr, const PointState<V, SPEC>& s,
+                                              int slot, bool single) {
+  VisWords w{0u, 0u, 0u};
+  if (V == kVis) {
+    const uint32_t* base = pr.vis + (int64_t)slot * pr.N_vertex;
+    w.w0 = base[s.vert0];
+    if (!single) {
+      w.w1 = base[s.vert1];
+      w.w2 = base[s.vert2];
+    }
+  }
+  return w;
+}
+
+// vis weight for bit b: sum_j c_j * bit_j (bs:2186-2190).  `single` is the
+// wave-uniform case where all three vertices of every lane coincide (one word
+// per slot, weight c0+c1+c2).
+template <int V, bool SPEC>
+__device__ __forceinline__ float vis_weight(const PointState<V, SPEC>& s, const VisWords& w, int b,
+                                            bool single, float csum) {
+  if (V != kVis) return s.live;
+  const float b0 = (float)__builtin_amdgcn_ubfe(w.w0, b, 1);
+  if (single) return b0 * csum;
+  const float b1 = (float)__builtin_amdgcn_ubfe(w.w1, b, 1);
+  const float b2 = (float)__builtin_amdgcn_ubfe(w.w2, b, 1);
+  return fmaf(b2, s.c2, fmaf(b1, s.c1, b0 * s.c0));
+}
+
+// ---- the per-pair lobe -------------------------------------------------------
+
+// Quantities of one (point, texel) pair shared by forward and backward.
+struct Lobe {
+  float cs;    // l.n
+  float nl;    // max(cs, 0)
+  float rs;    // 1/|l+v|
+  float ch;    // l.h
+  float nh;    // n.h
+  float om4;   // (1 - sat(ch))^4
+  float F, G, D;
+  float ia;    // 1/(nl(1-k)+k)
+  float it2;   // 1/max(pi tD^2, thres)
+  float tD;    // (n.h)^2 (alpha^2-1) + 1
+  float tmp;   // F G D / 4, or 0 where the reference skips the lobe (tmp <= 0 or NaN)
+  bool spec_on;
+};
+
+template <int V, bool SPEC>
+__device__ __forceinline__ Lobe eval_lobe(const PointState<V, SPEC>& s, const float4& g) {
+  Lobe o;
+  o.cs = fmaf(g.z, s.nz, fmaf(g.y, s.ny, g.x * s.nx));
+  o.nl = fmaxf(o.cs, 0.f);
+  if (SPEC) {
+    const float lv = fmaf(g.z, s.vz, fmaf(g.y, s.vy, g.x * s.vx));
+    // |l+v|^2 = |l|^2 + 2 l.v + |v|^2;  h = (l+v)/|l+v|
+    const float hb2 = fmaf(2.f, lv, g.w + s.vv);
+    // floor keeps (n.h)^2 finite when v = -l exactly; such pairs are masked below
+    o.rs = rsq(fmaxf(hb2, 1e-20f));
+    o.ch = (g.w + lv) * o.rs;
+    o.nh = (o.cs + s.nv) * o.rs;
+    const float om = 1.f - sat(o.ch);
+    const float om2 = om * om;
+    o.om4 = om2 * om2;
+    o.F = fmaf(o.om4 * om, 1.f - kF0, kF0);
+    o.ia = rcp(fmaf(o.nl, s.omk, s.k));
+    o.G = o.nl * o.ia * s.ib;
+    o.tD = fmaf(o.nh * o.nh, s.am1, 1.f);
+    o.it2 = rcp(fmaxf(kPi * o.tD * o.tD, kThres));
+    o.D = s.a2 * o.it2;
+    const float tmp = o.F * o.G * o.D * 0.25f;
+    // the reference skips the lobe unless tmp > 0; |l+v| = 0 gives NaN there
+    o.spec_on = (tmp > 0.f) && (hb2 > 0.f);
+    o.tmp = o.spec_on ? tmp : 0.f;
+  } else {
+    o.spec_on = false;
+    o.tmp = 0.f;
+  }
+  return o;
+}
+
+// ---- forward -----------------------------------------------------------------
+
+template <int V, bool DIFF, bool SPEC>
+__global__ __launch_bounds__(1024) void shade_forward_kernel(Problem pr, Tiling tl) {
+  extern __shared__ float4 smem[];
+  constexpr bool DOUT = (V == kV2 || V == kVis);
+  const int lane = threadIdx.x & 63;
+  const int wv = threadIdx.x >> 6;
+  const int64_t p = (int64_t)blockIdx.x * 64 + lane;
+  const bool valid = p < pr.P;
+
+  PointState<V, SPEC> st;
+  load_point<V, SPEC>(pr, p, valid, st);
+  bool single = true;
+  float csum = 0.f;
+  if (V == kVis) {
+    single = __all(st.vert0 == st.vert1 && st.vert1 == st.vert2);
+    csum = st.c0 + st.c1 + st.c2;
+  }
+
+  float ds0 = 0.f, ds1 = 0.f, ds2 = 0.f;  // sum E dOmega vis max(l.n,0)
+  float sp0 = 0.f, sp1 = 0.f, sp2 = 0.f;  // sum E dOmega vis F G D / 4
+
+  float4* geo = smem;
+  float4* rad = smem + tl.st;
+  for (int s0 = 0; s0 < pr.T; s0 += tl.st) {
+    const int sn = min(tl.st, pr.T - s0);
+    __syncthreads();
+    build_table(pr, s0, sn, tl.st, geo, rad);
+    __syncthreads();
+    const int c_begin = wv * tl.cw;
+    const int c_end = min(c_begin + tl.cw, (sn + 31) & ~31);
+    for (int sl = c_begin; sl < c_end; sl += 32) {
+      const VisWords w = load_slot<V, SPEC>(pr, st, (s0 + sl) >> 5, single);
+#pragma unroll 4
+      for (int b = 0; b < 32; ++b) {
+        const float4 g = geo[sl + b];
+        const float4 e = rad[sl + b];
+        const float vis = vis_weight<V, SPEC>(st, w, b, single, csum);
+        const Lobe lb = eval_lobe<V, SPEC>(st, g);
+        if (DIFF) {
+          const float x = lb.nl * vis;
+          ds0 = fmaf(e.x, x, ds0);
+          ds1 = fmaf(e.y, x, ds1);
+          ds2 = fmaf(e.z, x, ds2);
+        }
+        if (SPEC) {
+          const float y = lb.tmp * vis;
+          sp0 = fmaf(e.x, y, sp0);
+          sp1 = fmaf(e.y, y, sp1);
+          sp2 = fmaf(e.z, y, sp2);
+        }
+      }
+    }
+  }
+
+  // fixed-order reduction over the NTC texel chunks
+  constexpr int NACC = 6;
+  float acc[NACC] = {ds0, ds1, ds2, sp0, sp1, sp2};
+  if (tl.ntc > 1) {
+    __syncthreads();
+    float* red = reinterpret_cast<float*>(smem);
+#pragma unroll
+    for (int j = 0; j < NACC; ++j) red[(wv * NACC + j) * 64 + lane] = acc[j];
+    __syncthreads();
+    if (wv != 0) return;
+#pragma unroll
+    for (int j = 0; j < NACC; ++j) {
+      float sum = red[j * 64 + lane];
+      for (int w = 1; w < tl.ntc; ++w) sum += red[(w * NACC + j) * 64 + lane];
+      acc[j] = sum;
+    }
+  }
+  if (!valid) return;
+  float dsh[3], out[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    dsh[c] = acc[c] * kInvPi;
+    float o = 0.f;
+    if (DIFF) o = pr.albedo[3 * p + c] * dsh[c];
+    if (SPEC) o = fmaf(pr.spec[3 * p + c], acc[3 + c], o);
+    out[c] = o;
+  }
+#pragma unroll
+  for (int c = 0; c < 3; ++c) pr.sh[3 * p + c] = out[c];
+  if (DOUT) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) pr.ds[3 * p + c] = dsh[c];
+  }
+}
+
+// ---- backward ----------------------------------------------------------------
+//
+// Per pair (with x = vis max(l.n,0), y = vis F G D / 4, E'_c = E_c dOmega,
+// A_c = (gs_c a_c [+ gd_c]) / pi, B_c = gs_c s_c):
+//   grad_albedo_c   = gs_c / pi * sum E'_c x
+//   grad_spec_c     = gs_c * sum E'_c y
+//   gE_c(t)         = dOmega_t * sum_p (A_c x + B_c y)
+//   l-coefficients of grad_normal / grad_view_dir come from the diffuse term,
+//   the G1 VJP (bs:173-201), the NDF VJP (bs:212-227) and the normalize VJP
+//   (bs:62-71) expanded over h = (l+v) rs; their n- and v-directed parts are
+//   per-point constants times scalar sums (U, Q, Sb), applied once at the end.
+template <int V, bool DIFF, bool SPEC>
+__global__ __launch_bounds__(512) void shade_backward_kernel(Problem pr, Tiling tl) {
+  extern __shared__ float4 smem[];
+  constexpr bool DOUT = (V == kV2 || V == kVis);
+  const int lane = threadIdx.x & 63;
+  const int wv = threadIdx.x >> 6;
+  const int64_t p = (int64_t)blockIdx.x * 64 + lane;
+  const bool valid = p < pr.P;
+
+  PointState<V, SPEC> st;
+  load_point<V, SPEC>(pr, p, valid, st);
+  bool single = true;
+  float csum = 0.f;
+  if (V == kVis) {
+    single = __all(st.vert0 == st.vert1 && st.vert1 == st.vert2);
+    csum = st.c0 + st.c1 + st.c2;
+  }
+  float gs[3] = {0.f, 0.f, 0.f}, A[3] = {0.f, 0.f, 0.f}, B[3] = {0.f, 0.f, 0.f};
+  if (valid) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      gs[c] = pr.g_sh[3 * p + c];
+      if (DIFF) {
+        float up = gs[c] * pr.albedo[3 * p + c];
+        if (DOUT) up += pr.g_ds[3 * p + c];
+        A[c] = up * kInvPi;
+      }
+      if (SPEC) B[c] = gs[c] * pr.spec[3 * p + c];
+    }
+  }
+
+  float ds0 = 0.f, ds1 = 0.f, ds2 = 0.f;
+  float sp0 = 0.f, sp1 = 0.f, sp2 = 0.f;
+  float lnx = 0.f, lny = 0.f, lnz = 0.f;  // sum coef_n l
+  float lvx = 0.f, lvy = 0.f, lvz = 0.f;  // sum coef_v l
+  float U = 0.f, Q = 0.f, Sb = 0.f, ga2 = 0.f, gk = 0.f;
+
+  float4* geo = smem;
+  float4* rad = smem + tl.st;
+  float* part = pr.part + (int64_t)blockIdx.x * 3 * pr.T;
+  // lanes (lane & 15) < 3 of row k store channel (lane & 15) of texel kRowTexel[k]
+  const int row = lane >> 4;
+  const int row_texel = (row == 1) ? 2 : ((row == 2) ? 1 : row);
+  const int ch_out = lane & 15;
+
+  for (int s0 = 0; s0 < pr.T; s0 += tl.st) {
+    const int sn = min(tl.st, pr.T - s0);
+    __syncthreads();
+    build_table(pr, s0, sn, tl.st, geo, rad);
+    __syncthreads();
+    const int c_begin = wv * tl.cw;
+    const int c_end = min(c_begin + tl.cw, (sn + 31) & ~31);
+    for (int sl = c_begin; sl < c_end; sl += 32) {
+      const VisWords w = load_slot<V, SPEC>(pr, st, (s0 + sl) >> 5, single);
+#pragma unroll 1
+      for (int b4 = 0; b4 < 32; b4 += 4) {
+        float q[4][3];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int b = b4 + i;
+          const float4 g = geo[sl + b];
+          const float4 e = rad[sl + b];
+          const float vis = vis_weight<V, SPEC>(st, w, b, single, csum);
+          const Lobe lb = eval_lobe<V, SPEC>(st, g);
+          float coef_n = 0.f;
+          float x = 0.f, y = 0.f;
+          if (DIFF) {
+            x = lb.nl * vis;
+            ds0 = fmaf(e.x, x, ds0);
+            ds1 = fmaf(e.y, x, ds1);
+            ds2 = fmaf(e.z, x, ds2);
+            // gdot = vis [l.n > 0] sum_c A_c E'_c   (bs:2426-2438)
+            const float AE = fmaf(A[2], e.z, fmaf(A[1], e.y, A[0] * e.x));
+            coef_n = (lb.cs > 0.f) ? AE * vis : 0.f;
+          }
+          if (SPEC) {
+            y = lb.tmp * vis;
+            sp0 = fmaf(e.x, y, sp0);
+            sp1 = fmaf(e.y, y, sp1);
+            sp2 = fmaf(e.z, y, sp2);
+            const float BE = fmaf(B[2], e.z, fmaf(B[1], e.y, B[0] * e.x));
+            const float gt = lb.spec_on ? BE * vis : 0.f;  // grad of F G D / 4
+            const float gq = 0.25f * gt;
+            const float gF = gq * lb.G * lb.D;
+            const float gG = gq * lb.F * lb.D;
+            const float gD = gq * lb.F * lb.G;
+            // Fresnel VJP (bs:155-159): 5 (F0-1) g (1-sat c)^4 [0<=c<=1]
+            const float gch = (lb.ch >= 0.f && lb.ch <= 1.f) ? (5.f * (kF0 - 1.f)) * gF * lb.om4 : 0.f;
+            // G1 VJP (bs:173-201); l.n > 0 holds wherever gt != 0
+            const float grad_a = gG * st.ib;
+            const float grad_b = -gG * lb.G * st.ib;
+            const float ia2 = lb.ia * lb.ia;
+            const float gdl = grad_a * st.k * ia2;
+            gk = fmaf(grad_a * lb.nl * (lb.nl - 1.f), ia2, gk);
+            Sb += grad_b;
+            // NDF VJP (bs:212-227)
+            ga2 = fmaf(gD, lb.it2, ga2);
+            const bool big = kPi * lb.tD * lb.tD > kThres;
+            const float g_t = big ? -2.f * gD * lb.D * rcp(lb.tD) : 0.f;
+            const float nh2 = lb.nh * lb.nh;
+            ga2 = fmaf(nh2, g_t, ga2);
+            const float gdnh = 2.f * g_t * st.am1 * lb.nh;
+            // h = (l+v) rs; grad_h = gch l + gdnh n; normalize VJP:
+            //   rs (grad_h - h (h.grad_h)) = rs gch l + rs gdnh n - q (l + v)
+            const float rg = lb.rs * gdnh;
+            const float q2 = lb.rs * lb.rs * fmaf(gch, lb.ch, gdnh * lb.nh);
+            U += rg;
+            Q += q2;
+            coef_n += gdl + rg;
+            const float coef_v = fmaf(lb.rs, gch, -q2);
+            lvx = fmaf(coef_v, g.x, lvx);
+            lvy = fmaf(coef_v, g.y, lvy);
+            lvz = fmaf(coef_v, g.z, lvz);
+          }
+          lnx = fmaf(coef_n, g.x, lnx);
+          lny = fmaf(coef_n, g.y, lny);
+          lnz = fmaf(coef_n, g.z, lnz);
+#pragma unroll
+          for (int c = 0; c < 3; ++c) q[i][c] = fmaf(A[c], x, B[c] * y);
+        }
+        // env-map gradient: sum over the 64 points of this tile, 4 texels at a time
+        const float r0 = reduce4(q[0][0], q[1][0], q[2][0], q[3][0]);
+        const float r1 = reduce4(q[0][1], q[1][1], q[2][1], q[3][1]);
+        const float r2 = reduce4(q[0][2], q[1][2], q[2][2], q[3][2]);
+        const int t = s0 + sl + b4 + row_texel;
+        if (ch_out < 3 && t < pr.T) {
+          const float val = (ch_out == 0) ? r0 : ((ch_out == 1) ? r1 : r2);
+          part[(int64_t)ch_out * pr.T + t] = val;
+        }
+      }
+    }
+  }
+
+  // fixed-order reduction of the per-point sums over the NTC texel chunks
+  constexpr int NACC = 17;
+  float acc[NACC] = {ds0, ds1, ds2, sp0, sp1, sp2, lnx, lny, lnz,
+                     lvx, lvy, lvz, U,   Q,   Sb,  ga2, gk};
+  if (tl.ntc > 1) {
+    __syncthreads();
+    float* red = reinterpret_cast<float*>(smem);
+#pragma unroll
+    for (int j = 0; j < NACC; ++j) red[(wv * NACC + j) * 64 + lane] = acc[j];
+    __syncthreads();
+    if (wv != 0) return;
+#pragma unroll
+    for (int j = 0; j < NACC; ++j) {
+      float sum = red[j * 64 + lane];
+      for (int w = 1; w < tl.ntc; ++w) sum += red[(w * NACC + j) * 64 + lane];
+      acc[j] = sum;
+    }
+  }
+  if (!valid) return;
+  // gn = sum coef_n l + v (U + [n.v>0](1-k) Sb)
+  // gv = sum coef_v l + n (U + [n.v>0](1-k) Sb) - v Q
+  const float nvpos = (st.nv > 0.f) ? 1.f : 0.f;
+  const float common = SPEC ? fmaf(nvpos * st.omk, acc[14], acc[12]) : 0.f;
+  const float Qs = SPEC ? acc[13] : 0.f;
+  pr.gn[3 * p + 0] = fmaf(st.vx, common, acc[6]);
+  pr.gn[3 * p + 1] = fmaf(st.vy, common, acc[7]);
+  pr.gn[3 * p + 2] = fmaf(st.vz, common, acc[8]);
+  if (V != kDiffuse) {
+    // disabled branches are written as zeros (bs:2889-2905, bs:3205-3214)
+    const float gvx = fmaf(-st.vx, Qs, fmaf(st.nx, common, acc[9]));
+    const float gvy = fmaf(-st.vy, Qs, fmaf(st.ny, common, acc[10]));
+    const float gvz = fmaf(-st.vz, Qs, fmaf(st.nz, common, acc[11]));
+    pr.gv[3 * p + 0] = SPEC ? gvx : 0.f;
+    pr.gv[3 * p + 1] = SPEC ? gvy : 0.f;
+    pr.gv[3 * p + 2] = SPEC ? gvz : 0.f;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) pr.gs[3 * p + c] = SPEC ? gs[c] * acc[3 + c] : 0.f;
+    float gr = 0.f;
+    if (SPEC) {
+      const float r = pr.rough[p];
+      const float gkt = fmaf(1.f - fmaxf(st.nv, 0.f), acc[14], acc[16]);
+      // bs:2501-2502
+      gr = fmaf(acc[15] * 4.f * r * r, r, gkt * (1.f + r) * 0.25f);
+    }
+    pr.gr[p] = gr;
+  }
+#pragma unroll
+  for (int c = 0; c < 3; ++c) pr.ga[3 * p + c] = DIFF ? gs[c] * acc[c] * kInvPi : 0.f;
+}
+
+// ---- env-map gradient: fixed-order FP64 reduction of the per-tile partials -----
+
+// stage 1: rows [g*R, min((g+1)*R, nrows)) of part[nrows][ncol] -> part2[g][ncol]
+__global__ __launch_bounds__(256) void reduce_partials_kernel(const float* __restrict__ part,
+                                                              double* __restrict__ part2,
+                                                              int64_t nrows, int ncol, int R) {
+  const int col = blockIdx.x * 256 + threadIdx.x;
+  const int64_t g = blockIdx.y;
+  if (col >= ncol) return;
+  const int64_t r0 = g * R;
+  const int64_t r1 = min<int64_t>(r0 + R, nrows);
+  double s = 0.0;
+  int64_t r = r0;
+  for (; r + 4 <= r1; r += 4) {
+    const float a = part[r * ncol + col];
+    const float b = part[(r + 1) * ncol + col];
+    const float c = part[(r + 2) * ncol + col];
+    const float d = part[(r + 3) * ncol + col];
+    s += (double)a;
+    s += (double)b;
+    s += (double)c;
+    s += (double)d;
+  }
+  for (; r < r1; ++r) s += (double)part[r * ncol + col];
+  part2[g * ncol + col] = s;
+}
+
+// stage 2: gE[c][t] = dOmega_t * sum_g part2[g][c*T + t], rounded once to fp32
+__global__ __launch_bounds__(256) void finish_envgrad_kernel(const double* __restrict__ part2,
+                                                             float* __restrict__ gE, int ngroups,
+                                                             int H, int W) {
+  const int T = H * W;
+  const int col = blockIdx.x * 256 + threadIdx.x;
+  if (col >= 3 * T) return;
+  const int t = col % T;
+  const int iy = H - 1 - t / W;
+  double s = 0.0;
+  for (int g = 0; g < ngroups; ++g) s += part2[(int64_t)g * 3 * T + col];
+  const double dw = 4.0 * kPiD * sinpi(0.5 / (double)H) / (double)W *
+                    sinpi(((double)iy + 0.5) / (double)H);
+  gE[col] = (float)(s * dw);
+}
+
+__global__ __launch_bounds__(256) void fill_zero_kernel(float* __restrict__ x, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) x[i] = 0.f;
+}
+
+template <int V, bool DIFF, bool SPEC>
+hipError_t launch_fwd(const Problem& pr, const Tiling& tl, hipStream_t stream) {
+  const unsigned nblk = (unsigned)((pr.P + 63) / 64);
+  hipLaunchKernelGGL((shade_forward_kernel<V, DIFF, SPEC>), dim3(nblk), dim3(64 * tl.ntc),
+                     tl.lds_fwd, stream, pr, tl);
+  return hipGetLastError();
+}
+
+template <int V, bool DIFF, bool SPEC>
+hipError_t launch_bwd(const Problem& pr, const Tiling& tl, hipStream_t stream) {
+  const unsigned nblk = (unsigned)((pr.P + 63) / 64);
+  hipLaunchKernelGGL((shade_backward_kernel<V, DIFF, SPEC>), dim3(nblk), dim3(64 * tl.ntc),
+                     tl.lds_bwd, stream, pr, tl);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_forward(int variant, bool diff, bool spec, const Problem& pr, const Tiling& tl,
+                          hipStream_t stream) {
+  switch (variant) {
+    case kDiffuse: return launch_fwd<kDiffuse, true, false>(pr, tl, stream);
+    case kSpecular:
+      if (diff && spec) return launch_fwd<kSpecular, true, true>(pr, tl, stream);
+      if (diff) return launch_fwd<kSpecular, true, false>(pr, tl, stream);
+      if (spec) return launch_fwd<kSpecular, false, true>(pr, tl, stream);
+      return launch_fwd<kSpecular, false, false>(pr, tl, stream);
+    case kClamp:
+      if (diff && spec) return launch_fwd<kClamp, true, true>(pr, tl, stream);
+      if (diff) return launch_fwd<kClamp, true, false>(pr, tl, stream);
+      if (spec) return launch_fwd<kClamp, false, true>(pr, tl, stream);
+      return launch_fwd<kClamp, false, false>(pr, tl, stream);
+    case kV2:
+      if (spec) return launch_fwd<kV2, true, true>(pr, tl, stream);
+      return launch_fwd<kV2, true, false>(pr, tl, stream);
+    case kVis:
+      if (spec) return launch_fwd<kVis, true, true>(pr, tl, stream);
+      return launch_fwd<kVis, true, false>(pr, tl, stream);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_backward(int variant, bool diff, bool spec, const Problem& pr, const Tiling& tl,
+                           hipStream_t stream) {
+  hipError_t e;
+  switch (variant) {
+    case kDiffuse: e = launch_bwd<kDiffuse, true, false>(pr, tl, stream); break;
+    case kSpecular:
+      if (diff && spec) e = launch_bwd<kSpecular, true, true>(pr, tl, stream);
+      else if (diff) e = launch_bwd<kSpecular, true, false>(pr, tl, stream);
+      else if (spec) e = launch_bwd<kSpecular, false, true>(pr, tl, stream);
+      else e = launch_bwd<kSpecular, false, false>(pr, tl, stream);
+      break;
+    case kClamp:
+      if (diff && spec) e = launch_bwd<kClamp, true, true>(pr, tl, stream);
+      else if (diff) e = launch_bwd<kClamp, true, false>(pr, tl, stream);
+      else if (spec) e = launch_bwd<kClamp, false, true>(pr, tl, stream);
+      else e = launch_bwd<kClamp, false, false>(pr, tl, stream);
+      break;
+    case kV2:
+      e = spec ? launch_bwd<kV2, true, true>(pr, tl, stream) : launch_bwd<kV2, true, false>(pr, tl, stream);
+      break;
+    case kVis:
+      e = spec ? launch_bwd<kVis, true, true>(pr, tl, stream) : launch_bwd<kVis, true, false>(pr, tl, stream);
+      break;
+    default: return hipErrorInvalidValue;
+  }
+  return e;
+}
+
+hipError_t launch_envgrad_reduce(const Problem& pr, const Tiling& tl, hipStream_t stream) {
+  const int ncol = 3 * pr.T;
+  const int64_t nrows = (pr.P + 63) / 64;
+  const int ngroups = (int)((nrows + tl.reduce_rows - 1) / tl.reduce_rows);
+  const unsigned cb = (unsigned)((ncol + 255) / 256);
+  hipLaunchKernelGGL(reduce_partials_kernel, dim3(cb, ngroups), dim3(256), 0, stream, pr.part,
+                     pr.part2, nrows, ncol, tl.reduce_rows);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(finish_envgrad_kernel, dim3(cb), dim3(256), 0, stream, pr.part2, pr.gE,
+                     ngroups, pr.H, pr.W);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_zero(float* x, int64_t n, hipStream_t stream) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(fill_zero_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, x,
+                     n);
+  return hipGetLastError();
+}
+
+}  // namespace des

@@ -1,0 +1,193 @@
+/*
+ * diff_envshade.h -- C ABI of the MI355X (gfx950) brute-force environment-light
+ * shader.  Drop-in boundary for the ten shader bindings of zjumsj/diff_envshade
+ * (bindings.cpp:6-19, prototypes gsface_shader.h:3-118, host wrappers
+ * bruteforce_shader.cu:2687-3453).
+ *
+ * Conventions (all entry points):
+ *   - every pointer is a device pointer to contiguous row-major data on the
+ *     current HIP device; float = IEEE binary32;
+ *   - shapes: normal, view_dir, albedo, specular_albedo, every P x 3 output
+ *     and upstream gradient: P x 3; roughness and grad_roughness: P x 1;
+ *     envmap and grad_envmap: C x H x W (channel-major); light2obj_rotmat:
+ *     3 x 3 row-major; face_vertex_list: 3 x N_face int32;
+ *     nearest_triangle_id: P int64; barycentric_coord: 3 x P;
+ *     visibility: N_slot x N_vertex 32-bit words (bit t%32 of word
+ *     [t/32][vertex] is texel t, storage order);
+ *   - C must be 3 (DES_ERR_UNSUPPORTED_CHANNELS otherwise, as the reference's
+ *     "Channel number not supported!");
+ *   - `stream` is a hipStream_t (NULL = default stream); work is enqueued
+ *     asynchronously and nothing is allocated: outputs are caller-owned and
+ *     every output element is written (disabled branches as zeros), so the
+ *     caller may pass uninitialised buffers;
+ *   - the backward entry points need a caller-provided workspace of at least
+ *     des_backward_workspace_size(P, H, W) bytes (256-byte aligned) for the
+ *     per-tile env-map-gradient partials; grad_envmap is summed from them in a
+ *     fixed order in FP64, so it is bitwise reproducible run to run;
+ *   - return 0 (DES_OK) or a negative DES_ERR_* code; des_last_error() gives
+ *     the message of the calling thread's last failure.
+ *   - calls on distinct streams with distinct buffers are independent; calls
+ *     sharing a workspace must be ordered on one stream.
+ *
+ * Deliberate differences from the reference: launches go to the given stream
+ * (not the legacy default stream), shapes are validated (N_slot >= ceil(H*W/32)),
+ * grad_envmap is deterministic, and out-of-range face / vertex ids read as
+ * occluded instead of faulting.
+ */
+#ifndef DIFF_ENVSHADE_H
+#define DIFF_ENVSHADE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DES_API __attribute__((visibility("default")))
+
+enum {
+  DES_OK = 0,
+  DES_ERR_INVALID_ARGUMENT = -1,
+  DES_ERR_UNSUPPORTED_CHANNELS = -2,
+  DES_ERR_WORKSPACE_TOO_SMALL = -3,
+  DES_ERR_LAUNCH = -4
+};
+
+/* Version of this ABI (bumped on any signature change). */
+DES_API int des_abi_version(void);
+
+/* Message of the calling thread's most recent failed call ("" if none). */
+DES_API const char* des_last_error(void);
+
+/* Workspace bytes needed by every *_backward* entry point for P points and an
+ * H x W env map (0 when P == 0). */
+DES_API size_t des_backward_workspace_size(int64_t P, int32_t H, int32_t W);
+
+/* bindings.cpp:6 -> bruteforce_diffuse_shader_forward (bruteforce_shader.cu:2687-2731).
+ * Lambert only: shading = sum_t [l.n>0] E a (l.n/pi) dOmega. */
+DES_API int des_bruteforce_diffuse_shader_forward(
+    int64_t P, int32_t C, int32_t H, int32_t W,
+    const float* normal, const float* albedo, const float* envmap,
+    const float* light2obj_rotmat,
+    float* shading, void* stream);
+
+/* bindings.cpp:7 -> bruteforce_diffuse_shader_backward (bruteforce_shader.cu:2733-2792).
+ * Outputs [grad_normal, grad_albedo, grad_envmap]. */
+DES_API int des_bruteforce_diffuse_shader_backward(
+    int64_t P, int32_t C, int32_t H, int32_t W,
+    const float* grad_shading, const float* normal, const float* albedo,
+    const float* envmap, const float* light2obj_rotmat,
+    float* grad_normal, float* grad_albedo, float* grad_envmap,
+    void* workspace, size_t workspace_bytes, void* stream);
+
+/* bindings.cpp:9 -> bruteforce_specular_shader_forward (bruteforce_shader.cu:2796-2851).
+ * Lambert (if enable_diffuse) + GGX (if enable_specular). */
+DES_API int des_bruteforce_specular_shader_forward(
+    int64_t P, int32_t C, int32_t H, int32_t W,
+    const float* normal, const float* view_dir, const float* albedo,
+    const float* specular_albedo, const float* roughness, const float* envmap,
+    const float* light2obj_rotmat, int enable_diffuse, int enable_specular,
+    float* shading, void* stream);
+
+/* bindings.cpp:10 -> bruteforce_specular_shader_backward (bruteforce_shader.cu:2853-2949).
+ * Outputs [grad_normal, grad_view_dir, grad_albedo, grad_specular_albedo,
+ * grad_roughness, grad_envmap]; grad_albedo is zero if !enable_diffuse,
+ * grad_view_dir / grad_specular_albedo / grad_roughness zero if !enable_specular. */
+DES_API int des_bruteforce_specular_shader_backward(
+    int64_t P, int32_t C, int32_t H, int32_t W,
+    const float* grad_shading, const float* normal, const float* view_dir,
+    const float* albedo, const float* specular_albedo, const float* roughness,
+    const float* envmap, const float* light2obj_rotmat,
+    int enable_diffuse, int enable_specular,
+    float* grad_normal, float* grad_view_dir, float* grad_albedo,
+    float* grad_specular_albedo, float* grad_roughness, float* grad_envmap,
+    void* workspace, size_t workspace_bytes, void* stream);
+
+/* bindings.cpp:12 -> bruteforce_specular_shader_clamp_forward (bruteforce_shader.cu:2953-3008).
+ * As the specular forward, but points with n.v <= 0 shade to 0. */
+DES_API int des_bruteforce_specular_shader_clamp_forward(
+    int64_t P, int32_t C, int32_t H, int32_t W,
+    const float* normal, const float* view_dir, const float* albedo,
+    const float* specular_albedo, const float* roughness, const float* envmap,
+    const float* light2obj_rotmat, int enable_diffuse, int enable_specular,
+    float* shading, void* stream);
+
+/* bindings.cpp:13 -> bruteforce_specular_shader_clamp_backward (bruteforce_shader.cu:3010-3106). */
+DES_API int des_bruteforce_specular_shader_clamp_backward(
+    int64_t P, int32_t C, int32_t H, int32_t W,
+    const float* grad_shading, const float* normal, const float* view_dir,
+    const float* albedo, const float* specular_albedo, const float* roughness,
+    const float* envmap, const float* light2obj_rotmat,
+    int enable_diffuse, int enable_specular,
+    float* grad_normal, float* grad_view_dir, float* grad_albedo,
+    float* grad_specular_albedo, float* grad_roughness, float* grad_envmap,
+    void* workspace, size_t workspace_bytes, void* stream);
+
+/* bindings.cpp:15 -> bruteforce_specular_shader_forward2 (bruteforce_shader.cu:3108-3165).
+ * Diffuse always on; outputs [shading, diffuse_shading]. */
+DES_API int des_bruteforce_specular_shader_forward2(
+    int64_t P, int32_t C, int32_t H, int32_t W,
+    const float* normal, const float* view_dir, const float* albedo,
+    const float* specular_albedo, const float* roughness, const float* envmap,
+    const float* light2obj_rotmat, int enable_specular,
+    float* shading, float* diffuse_shading, void* stream);
+
+/* bindings.cpp:16 -> bruteforce_specular_shader_backward2 (bruteforce_shader.cu:3167-3259). */
+DES_API int des_bruteforce_specular_shader_backward2(
+    int64_t P, int32_t C, int32_t H, int32_t W,
+    const float* grad_shading, const float* grad_diffuse_shading,
+    const float* normal, const float* view_dir, const float* albedo,
+    const float* specular_albedo, const float* roughness, const float* envmap,
+    const float* light2obj_rotmat, int enable_specular,
+    float* grad_normal, float* grad_view_dir, float* grad_albedo,
+    float* grad_specular_albedo, float* grad_roughness, float* grad_envmap,
+    void* workspace, size_t workspace_bytes, void* stream);
+
+/* bindings.cpp:18 -> bruteforce_specularvisibility_shader_forward2
+ * (bruteforce_shader.cu:3261-3336).  v2 times barycentric-interpolated
+ * per-vertex visibility bits. */
+DES_API int des_bruteforce_specularvisibility_shader_forward2(
+    int64_t P, int32_t C, int32_t H, int32_t W,
+    int32_t N_face, int32_t N_vertex, int32_t N_slot,
+    const float* normal, const float* view_dir, const float* albedo,
+    const float* specular_albedo, const float* roughness, const float* envmap,
+    const float* light2obj_rotmat,
+    const int32_t* face_vertex_list, const int64_t* nearest_triangle_id,
+    const float* barycentric_coord, const int32_t* visibility,
+    int enable_specular,
+    float* shading, float* diffuse_shading, void* stream);
+
+/* bindings.cpp:19 -> bruteforce_specularvisibility_shader_backward2
+ * (bruteforce_shader.cu:3338-3453). */
+DES_API int des_bruteforce_specularvisibility_shader_backward2(
+    int64_t P, int32_t C, int32_t H, int32_t W,
+    int32_t N_face, int32_t N_vertex, int32_t N_slot,
+    const float* grad_shading, const float* grad_diffuse_shading,
+    const float* normal, const float* view_dir, const float* albedo,
+    const float* specular_albedo, const float* roughness, const float* envmap,
+    const float* light2obj_rotmat,
+    const int32_t* face_vertex_list, const int64_t* nearest_triangle_id,
+    const float* barycentric_coord, const int32_t* visibility,
+    int enable_specular,
+    float* grad_normal, float* grad_view_dir, float* grad_albedo,
+    float* grad_specular_albedo, float* grad_roughness, float* grad_envmap,
+    void* workspace, size_t workspace_bytes, void* stream);
+
+/* Launch-shape override for tuning (no counterpart in the reference): number
+ * of waves splitting the texel range per 64-point tile, forward and backward
+ * (each a power of two, forward <= 16, backward <= 8; 0 restores the built-in
+ * choice).  Process-wide; results do not depend on it beyond fp32 rounding. */
+DES_API int des_set_tiling_override(int32_t ntc_forward, int32_t ntc_backward);
+
+/* Timing hook for benchmarks (no counterpart in the reference): when set,
+ * the calling thread's next *_backward* call records hipEvent_t `start` right
+ * before and `stop` right after its main shading kernel, on the call's stream,
+ * then clears the hook.  Pass NULLs to clear it explicitly. */
+DES_API int des_set_backward_timing_events(void* start, void* stop);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DIFF_ENVSHADE_H */

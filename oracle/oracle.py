@@ -1,0 +1,160 @@
+"""ctypes wrapper around the FP64 oracle (oracle/des_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg as the checker / CPU baseline.  The product
+package never imports this module.
+
+All float inputs are widened to float64 (exactly) before the call; outputs are
+float64.  Argument meaning follows the reference bindings
+(gsface_shader.h:3-118 of zjumsj/diff_envshade).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle_des.so")
+
+VARIANTS = {"diffuse": 0, "specular": 1, "clamp": 2, "v2": 3, "vis": 4}
+
+_lib = None
+
+
+def build(force: bool = False) -> str:
+    """Compile the oracle with its Makefile (gcc + OpenMP)."""
+    if force or not os.path.exists(_LIB_PATH) or (
+        os.path.getmtime(_LIB_PATH) < os.path.getmtime(os.path.join(_HERE, "des_oracle.c"))
+    ):
+        subprocess.run(["make", "-B" if force else "-s", "-C", _HERE, "liboracle_des.so"],
+                       check=True, stdout=subprocess.DEVNULL)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        dp = ctypes.POINTER(ctypes.c_double)
+        i32p = ctypes.POINTER(ctypes.c_int32)
+        i64p = ctypes.POINTER(ctypes.c_int64)
+        u32p = ctypes.POINTER(ctypes.c_uint32)
+        c_int, c_i64 = ctypes.c_int, ctypes.c_int64
+        L.des_oracle_forward.restype = c_int
+        L.des_oracle_forward.argtypes = [c_int, c_int, c_int, c_i64, c_int, c_int, c_int, c_int,
+                                         dp, dp, dp, dp, dp, dp, dp, i32p, i64p, dp, u32p,
+                                         dp, dp, c_int]
+        L.des_oracle_backward.restype = c_int
+        L.des_oracle_backward.argtypes = [c_int, c_int, c_int, c_i64, c_int, c_int, c_int, c_int,
+                                          dp, dp, dp, dp, dp, dp, dp, dp, dp, i32p, i64p, dp,
+                                          u32p, dp, dp, dp, dp, dp, dp, c_int]
+        L.des_oracle_texel_table.restype = None
+        L.des_oracle_texel_table.argtypes = [c_int, c_int, dp, dp, dp]
+        L.des_oracle_delta_omega.restype = ctypes.c_double
+        L.des_oracle_delta_omega.argtypes = [c_int, c_int, c_int]
+        L.des_oracle_max_threads.restype = c_int
+        L.des_oracle_max_threads.argtypes = []
+        _lib = L
+    return _lib
+
+
+def _d(x):
+    if x is None:
+        return None, None
+    a = np.ascontiguousarray(np.asarray(x, dtype=np.float64))
+    return a, a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _typed(x, dtype, ctype):
+    if x is None:
+        return None, None
+    a = np.ascontiguousarray(np.asarray(x).astype(dtype, copy=False))
+    return a, a.ctypes.data_as(ctypes.POINTER(ctype))
+
+
+def max_threads() -> int:
+    return int(lib().des_oracle_max_threads())
+
+
+def texel_table(H: int, W: int, rotmat=None):
+    """Rotated texel directions (T x 3) and solid angles (T,) in FP64."""
+    M = np.eye(3) if rotmat is None else np.asarray(rotmat, dtype=np.float64)
+    M, Mp = _d(M)
+    T = H * W
+    l = np.zeros((T, 3))
+    dw = np.zeros(T)
+    lib().des_oracle_texel_table(H, W, Mp, l.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                 dw.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    return l, dw
+
+
+def _common(inp):
+    normal = inp["normal"]
+    P = int(np.asarray(normal).shape[0])
+    env = np.asarray(inp["envmap"])
+    C, H, W = env.shape
+    if C != 3:
+        raise RuntimeError("Channel number not supported!")
+    keep = []
+    ptrs = []
+    for k in ("normal", "view_dir", "albedo", "specular_albedo", "roughness", "envmap",
+              "light2obj_rotmat"):
+        a, p = _d(inp.get(k))
+        keep.append(a)
+        ptrs.append(p)
+    fvl, fvlp = _typed(inp.get("face_vertex_list"), np.int32, ctypes.c_int32)
+    tri, trip = _typed(inp.get("nearest_triangle_id"), np.int64, ctypes.c_int64)
+    bary, baryp = _d(inp.get("barycentric_coord"))
+    vis = inp.get("visibility")
+    if vis is not None:
+        vis = np.ascontiguousarray(np.asarray(vis).view(np.uint32) if np.asarray(vis).dtype
+                                   == np.int32 else np.asarray(vis, dtype=np.uint32))
+        visp = vis.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))
+    else:
+        visp = None
+    keep += [fvl, tri, bary, vis]
+    N_face = int(fvl.shape[1]) if fvl is not None else 0
+    N_vertex = int(vis.shape[1]) if vis is not None else 0
+    return P, H, W, N_face, N_vertex, ptrs + [fvlp, trip, baryp, visp], keep
+
+
+def forward(variant: str, inp: dict, enable_diffuse: bool = True, enable_specular: bool = True,
+            nthreads: int = 0):
+    """Returns (shading, diffuse_shading or None) as float64 arrays (P x 3)."""
+    v = VARIANTS[variant]
+    P, H, W, NF, NV, ptrs, keep = _common(inp)
+    sh = np.zeros((P, 3))
+    ds = np.zeros((P, 3)) if variant in ("v2", "vis") else None
+    dp = ctypes.POINTER(ctypes.c_double)
+    rc = lib().des_oracle_forward(v, int(enable_diffuse), int(enable_specular), P, H, W, NF, NV,
+                                  *ptrs, sh.ctypes.data_as(dp),
+                                  ds.ctypes.data_as(dp) if ds is not None else None, nthreads)
+    if rc != 0:
+        raise RuntimeError(f"oracle forward failed ({rc})")
+    return sh, ds
+
+
+def backward(variant: str, inp: dict, grad_shading, grad_diffuse_shading=None,
+             enable_diffuse: bool = True, enable_specular: bool = True, nthreads: int = 0):
+    """Returns dict of float64 grads: normal, view_dir, albedo, specular_albedo,
+    roughness (P x 1) and envmap (3 x H x W)."""
+    v = VARIANTS[variant]
+    P, H, W, NF, NV, ptrs, keep = _common(inp)
+    gsh, gshp = _d(grad_shading)
+    gds, gdsp = _d(grad_diffuse_shading)
+    out = {k: np.zeros((P, 3)) for k in ("normal", "view_dir", "albedo", "specular_albedo")}
+    out["roughness"] = np.zeros((P, 1))
+    out["envmap"] = np.zeros((3, H, W))
+    dp = ctypes.POINTER(ctypes.c_double)
+    rc = lib().des_oracle_backward(
+        v, int(enable_diffuse), int(enable_specular), P, H, W, NF, NV, gshp, gdsp, *ptrs,
+        *(out[k].ctypes.data_as(dp) for k in ("normal", "view_dir", "albedo", "specular_albedo",
+                                              "roughness", "envmap")), nthreads)
+    if rc != 0:
+        raise RuntimeError(f"oracle backward failed ({rc})")
+    return out

@@ -1,0 +1,115 @@
+"""Shared helpers for the parity tests: run one variant through the HIP drop-in
+and through the FP64 oracle on the same inputs, and compare.
+
+Error metric: normwise relative error per output tensor,
+    err(x, ref) = max|x - ref| / max|ref|,
+which stays meaningful for signed sums such as grad_normal (elementwise
+relative error is unbounded near zero crossings).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from diff_envshade_amd import workloads as WL
+from oracle import oracle as O
+
+# colors and every gradient must match the FP64 oracle to this normwise relative error
+TOL = 1e-5
+
+FWD_NAMES = {"diffuse": ["shading"], "specular": ["shading"], "clamp": ["shading"],
+             "v2": ["shading", "diffuse_shading"], "vis": ["shading", "diffuse_shading"]}
+BWD_NAMES = {"diffuse": ["normal", "albedo", "envmap"]}
+for _v in ("specular", "clamp", "v2", "vis"):
+    BWD_NAMES[_v] = ["normal", "view_dir", "albedo", "specular_albedo", "roughness", "envmap"]
+
+
+def relerr(x, ref) -> float:
+    x = np.asarray(x, dtype=np.float64)
+    ref = np.asarray(ref, dtype=np.float64)
+    assert x.shape == ref.shape, (x.shape, ref.shape)
+    den = np.abs(ref).max() if ref.size else 0.0
+    num = np.abs(x - ref).max() if ref.size else 0.0
+    return float(num / den) if den > 0 else float(num)
+
+
+def problem(seed=0, n=300, H=4, W=8, rotation=True, vis="none", p_vis=0.7, n_bright=0,
+            sigma=1.0):
+    """Random inputs. vis: 'none' | 'ones' | 'identity' | 'mesh'."""
+    rng = np.random.default_rng(seed)
+    d = WL.make_points(rng, n)
+    d["envmap"] = WL.make_envmap(rng, H, W, sigma, n_bright=n_bright)
+    d["light2obj_rotmat"] = WL.random_rotation(rng) if rotation else np.eye(3, dtype=np.float32)
+    d.update(WL.upstream_grads(rng, n))
+    if vis == "ones":
+        d["visibility"] = np.full(((H * W + 31) // 32, n), -1, dtype=np.int32)
+        d.update(WL.identity_mesh(n))
+    elif vis == "identity":
+        d["visibility"] = WL.make_visibility(rng, d["normal"], H, W, d["light2obj_rotmat"], p_vis)
+        d.update(WL.identity_mesh(n))
+    elif vis == "mesh":
+        nv = max(1, n // 2)
+        nf = max(1, n // 3)
+        words = rng.integers(0, 2**32, size=((H * W + 31) // 32, nv), dtype=np.uint64)
+        d["visibility"] = words.astype(np.uint32).view(np.int32)
+        d.update(WL.random_mesh(rng, n, nv, nf))
+    return d
+
+
+def oracle_run(variant, d, diff=True, spec=True):
+    sh, ds = O.forward(variant, d, diff, spec)
+    fwd = [sh] if ds is None else [sh, ds]
+    g = O.backward(variant, d, d["grad_shading"], d.get("grad_diffuse_shading"), diff, spec)
+    bwd = [g[k] for k in BWD_NAMES[variant]]
+    return fwd, bwd
+
+
+def gpu_inputs(d, device="cuda"):
+    import torch
+    return {k: torch.from_numpy(np.ascontiguousarray(v)).to(device) for k, v in d.items()}
+
+
+def gpu_run(variant, d, diff=True, spec=True, device="cuda"):
+    """Forward + backward through the drop-in bindings; returns numpy lists."""
+    import torch
+    from diff_envshade_amd import _diff_envshade as C
+    t = gpu_inputs(d, device)
+    mat = [t["normal"], t["view_dir"], t["albedo"], t["specular_albedo"], t["roughness"],
+           t["envmap"], t["light2obj_rotmat"]]
+    if variant == "diffuse":
+        fwd = [C.bruteforce_diffuse_shader_forward(t["normal"], t["albedo"], t["envmap"],
+                                                   t["light2obj_rotmat"])]
+        bwd = C.bruteforce_diffuse_shader_backward(t["grad_shading"], t["normal"], t["albedo"],
+                                                   t["envmap"], t["light2obj_rotmat"])
+    elif variant == "specular":
+        fwd = [C.bruteforce_specular_shader_forward(*mat, diff, spec)]
+        bwd = C.bruteforce_specular_shader_backward(t["grad_shading"], *mat, diff, spec)
+    elif variant == "clamp":
+        fwd = [C.bruteforce_specular_shader_clamp_forward(*mat, diff, spec)]
+        bwd = C.bruteforce_specular_shader_clamp_backward(t["grad_shading"], *mat, diff, spec)
+    elif variant == "v2":
+        fwd = C.bruteforce_specular_shader_forward2(*mat, spec)
+        bwd = C.bruteforce_specular_shader_backward2(t["grad_shading"],
+                                                     t["grad_diffuse_shading"], *mat, spec)
+    elif variant == "vis":
+        vis = [t["face_vertex_list"], t["nearest_triangle_id"], t["barycentric_coord"],
+               t["visibility"]]
+        fwd = C.bruteforce_specularvisibility_shader_forward2(*mat, *vis, spec)
+        bwd = C.bruteforce_specularvisibility_shader_backward2(
+            t["grad_shading"], t["grad_diffuse_shading"], *mat, *vis, spec)
+    else:
+        raise ValueError(variant)
+    torch.cuda.synchronize()
+    return [x.cpu().numpy() for x in fwd], [x.cpu().numpy() for x in bwd]
+
+
+def compare(variant, got, ref, tol=TOL):
+    """Assert every output within tol; returns {name: err}."""
+    (gf, gb), (rf, rb) = got, ref
+    errs = {}
+    for name, x, r in zip(FWD_NAMES[variant], gf, rf):
+        errs[name] = relerr(x, r)
+    for name, x, r in zip(BWD_NAMES[variant], gb, rb):
+        errs["grad_" + name] = relerr(x.reshape(r.shape), r)
+    bad = {k: v for k, v in errs.items() if not v <= tol}
+    assert not bad, f"{variant}: outputs beyond {tol}: {bad} (all: {errs})"
+    return errs

@@ -1,0 +1,116 @@
+"""C-ABI library: loads, exports every symbol declared in include/*.h, and
+rejects invalid arguments with the documented codes before touching a GPU
+(CPU only: no call here reaches a kernel launch)."""
+import ctypes
+import glob
+import os
+import re
+
+import pytest
+
+from diff_envshade_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DUMMY = 4096  # non-NULL placeholder pointer; never dereferenced on these paths
+
+
+def declared_symbols():
+    names = []
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        src = open(h).read()
+        names += re.findall(r"DES_API\s+[\w\s\*]+?\b(des_\w+)\s*\(", src)
+    return sorted(set(names))
+
+
+def test_header_declares_the_ten_shader_entry_points():
+    names = declared_symbols()
+    for v in ("diffuse_shader_forward", "diffuse_shader_backward", "specular_shader_forward",
+              "specular_shader_backward", "specular_shader_clamp_forward",
+              "specular_shader_clamp_backward", "specular_shader_forward2",
+              "specular_shader_backward2", "specularvisibility_shader_forward2",
+              "specularvisibility_shader_backward2"):
+        assert f"des_bruteforce_{v}" in names
+
+
+def test_library_exports_every_declared_symbol():
+    L = _lib.load()
+    missing = [n for n in declared_symbols() if not hasattr(L, n)]
+    assert not missing, missing
+    assert L.des_abi_version() == 1
+
+
+def test_library_links_no_torch():
+    # the boundary library depends on the HIP runtime only
+    with open(_lib.lib_path(), "rb") as f:
+        blob = f.read()
+    assert b"libamdhip64" in blob
+    assert b"libtorch" not in blob and b"libc10" not in blob
+
+
+def test_workspace_size():
+    assert _lib.workspace_size(0, 16, 32) == 0
+    n = _lib.workspace_size(70000, 16, 32)
+    tiles = (70000 + 63) // 64
+    assert n >= tiles * 3 * 512 * 4
+    assert n % 256 == 0
+    assert _lib.workspace_size(1, 16, 32) > 0
+
+
+def _vis_fwd(P=10, C=3, H=4, W=8, NF=10, NV=10, NS=1, ptr=DUMMY):
+    L = _lib.load()
+    return L.des_bruteforce_specularvisibility_shader_forward2(
+        P, C, H, W, NF, NV, NS, *([ptr] * 11), 1, ptr, ptr, None)
+
+
+def test_channels_must_be_three():
+    assert _vis_fwd(C=4) == -2
+    assert "Channel number not supported!" in _lib.load().des_last_error().decode()
+
+
+@pytest.mark.parametrize("kw,code", [({"P": -1}, -1), ({"H": 0}, -1), ({"W": -3}, -1),
+                                     ({"NS": 0}, -1), ({"NF": 0}, -1), ({"NV": 0}, -1),
+                                     ({"ptr": None}, -1)])
+def test_invalid_arguments(kw, code):
+    assert _vis_fwd(**kw) == code
+    assert _lib.load().des_last_error().decode() != ""
+
+
+def test_slots_must_cover_envmap():
+    # 16 x 32 = 512 texels need 16 slots of 32 bits
+    assert _vis_fwd(H=16, W=32, NS=15) == -1
+    assert "needs 16" in _lib.load().des_last_error().decode()
+
+
+def test_backward_rejects_small_workspace():
+    L = _lib.load()
+    P, H, W = 100, 4, 8
+    need = _lib.workspace_size(P, H, W)
+    rc = L.des_bruteforce_specular_shader_backward2(
+        P, 3, H, W, *([DUMMY] * 9), 1, *([DUMMY] * 6), DUMMY, need - 1, None)
+    assert rc == -3
+    rc = L.des_bruteforce_specular_shader_backward2(
+        P, 3, H, W, *([DUMMY] * 9), 1, *([DUMMY] * 6), None, need, None)
+    assert rc == -3
+    # misaligned workspace
+    rc = L.des_bruteforce_specular_shader_backward2(
+        P, 3, H, W, *([DUMMY] * 9), 1, *([DUMMY] * 6), DUMMY + 4, need, None)
+    assert rc == -1
+
+
+def test_tiling_override_validation():
+    with pytest.raises(RuntimeError):
+        _lib.set_tiling_override(3, 0)
+    with pytest.raises(RuntimeError):
+        _lib.set_tiling_override(0, 16)
+    _lib.set_tiling_override(0, 0)
+
+
+def test_error_raises_runtime_error_with_message():
+    with pytest.raises(RuntimeError, match="Channel number not supported!"):
+        _lib.check(_vis_fwd(C=1))
+
+
+def test_timing_events_need_both():
+    L = _lib.load()
+    assert L.des_set_backward_timing_events(ctypes.c_void_p(DUMMY), None) == -1
+    assert L.des_set_backward_timing_events(None, None) == 0
