@@ -8,7 +8,9 @@
 #include <atomic>
 #include <cstdarg>
 #include <cstdio>
+#include <cstring>
 #include <string>
+#include <vector>
 
 #include "../../include/diff_envshade.h"
 #include "envshade_internal.h"
@@ -44,6 +46,22 @@ int64_t n_tiles(int64_t P) { return (P + kTileP - 1) / kTileP; }
 
 bool is_pow2(int x) { return x > 0 && (x & (x - 1)) == 0; }
 
+des::Tiling tiling_for(int ntc, int T, bool backward) {
+  const int nslot = (T + 31) / 32;
+  const int max_slots = des::kMaxSuperChunk / 32;
+  const int slots_per_wave = std::max(1, std::min((nslot + ntc - 1) / ntc, max_slots / ntc));
+  des::Tiling tl;
+  tl.ntc = ntc;
+  tl.cw = slots_per_wave * 32;
+  tl.st = ntc * tl.cw;
+  const size_t table = size_t(32) * tl.st;
+  tl.lds_fwd = std::max(table, size_t(ntc) * des::kForwardAcc * 64 * sizeof(float));
+  tl.lds_bwd = std::max(table, size_t(ntc) * des::kBackwardAcc * 64 * sizeof(float));
+  tl.reduce_rows = des::kReduceRows;
+  (void)backward;
+  return tl;
+}
+
 // Launch shape as a function of (P, T) only, so results never depend on the
 // device it runs on.
 des::Tiling choose_tiling(int64_t P, int T, bool backward) {
@@ -57,17 +75,7 @@ des::Tiling choose_tiling(int64_t P, int T, bool backward) {
   } else {
     while (ntc * 2 <= cap && ntc * 2 <= nslot && tiles * ntc < kTargetWaves) ntc *= 2;
   }
-  const int max_slots = des::kMaxSuperChunk / 32;
-  const int slots_per_wave = std::max(1, std::min((nslot + ntc - 1) / ntc, max_slots / ntc));
-  des::Tiling tl;
-  tl.ntc = ntc;
-  tl.cw = slots_per_wave * 32;
-  tl.st = ntc * tl.cw;
-  const size_t table = size_t(32) * tl.st;
-  tl.lds_fwd = std::max(table, size_t(ntc) * des::kForwardAcc * 64 * sizeof(float));
-  tl.lds_bwd = std::max(table, size_t(ntc) * des::kBackwardAcc * 64 * sizeof(float));
-  tl.reduce_rows = des::kReduceRows;
-  return tl;
+  return tiling_for(ntc, T, backward);
 }
 
 size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
@@ -169,6 +177,101 @@ des::Problem make_problem(int64_t P, int32_t H, int32_t W) {
   pr.W = W;
   pr.T = H * W;
   return pr;
+}
+
+
+// ---- batches ------------------------------------------------------------------
+
+constexpr size_t kHeaderBytes = 256;
+static_assert(sizeof(des::BatchHeader) <= kHeaderBytes, "header too large");
+
+struct BatchLayout {
+  size_t sets_off, table_bytes, total;
+  std::vector<size_t> part_off, part2_off;
+};
+
+BatchLayout batch_layout(const des_set* sets, int n) {
+  BatchLayout L;
+  L.sets_off = kHeaderBytes;
+  L.table_bytes = align256(kHeaderBytes + sizeof(des::BatchSet) * size_t(n));
+  size_t off = L.table_bytes;
+  L.part_off.resize(n);
+  L.part2_off.resize(n);
+  for (int i = 0; i < n; ++i) {
+    const WsLayout w = ws_layout(sets[i].P, sets[i].H * sets[i].W);
+    L.part_off[i] = off + w.part_off;
+    L.part2_off[i] = off + w.part2_off;
+    off += w.total;
+  }
+  L.total = off;
+  return L;
+}
+
+int check_set(int variant, bool diff, bool spec, bool with_bwd, const des_set& s, int i) {
+  if (int rc = check_common(s.P, 3, s.H, s.W))
+    return fail(rc, "set %d: %s", i, g_last_error.c_str());
+  if (s.P == 0) {
+    if (with_bwd && s.grad_envmap == nullptr) return fail(DES_ERR_INVALID_ARGUMENT, "set %d: grad_envmap must not be NULL", i);
+    return DES_OK;
+  }
+  const bool dout = variant == des::kV2 || variant == des::kVis;
+  if (variant == des::kVis) {
+    if (int rc = check_vis(s.P, s.H * s.W, s.N_face, s.N_vertex, s.N_slot))
+      return fail(rc, "set %d: %s", i, g_last_error.c_str());
+    if (!s.face_vertex_list || !s.nearest_triangle_id || !s.barycentric_coord || !s.visibility)
+      return fail(DES_ERR_INVALID_ARGUMENT, "set %d: visibility inputs must not be NULL", i);
+  }
+  if (!s.normal || !s.envmap || !s.light2obj_rotmat || !s.shading)
+    return fail(DES_ERR_INVALID_ARGUMENT, "set %d: normal, envmap, light2obj_rotmat and shading must not be NULL", i);
+  if (diff && !s.albedo) return fail(DES_ERR_INVALID_ARGUMENT, "set %d: albedo must not be NULL", i);
+  if ((spec || variant == des::kClamp) && !s.view_dir)
+    return fail(DES_ERR_INVALID_ARGUMENT, "set %d: view_dir must not be NULL", i);
+  if (spec && (!s.specular_albedo || !s.roughness))
+    return fail(DES_ERR_INVALID_ARGUMENT, "set %d: specular_albedo and roughness must not be NULL", i);
+  if (dout && !s.diffuse_shading)
+    return fail(DES_ERR_INVALID_ARGUMENT, "set %d: diffuse_shading must not be NULL", i);
+  if (with_bwd) {
+    if (!s.grad_shading || !s.grad_normal || !s.grad_albedo || !s.grad_envmap)
+      return fail(DES_ERR_INVALID_ARGUMENT, "set %d: backward buffers must not be NULL", i);
+    if (variant != des::kDiffuse && (!s.grad_view_dir || !s.grad_specular_albedo || !s.grad_roughness))
+      return fail(DES_ERR_INVALID_ARGUMENT, "set %d: backward buffers must not be NULL", i);
+    if (dout && !s.grad_diffuse_shading)
+      return fail(DES_ERR_INVALID_ARGUMENT, "set %d: grad_diffuse_shading must not be NULL", i);
+  }
+  return DES_OK;
+}
+
+int check_ntc(int ntc, int cap, const char* what) {
+  if (ntc < 1 || ntc > cap || !is_pow2(ntc))
+    return fail(DES_ERR_INVALID_ARGUMENT, "%s must be a power of two in [1, %d] (got %d)", what, cap, ntc);
+  return DES_OK;
+}
+
+int num_cus() {
+  static int cached[64] = {0};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+  if (cached[dev] == 0) {
+    int n = 0;
+    if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) n = 256;
+    cached[dev] = n;
+  }
+  return cached[dev];
+}
+
+des::BatchLaunch batch_launch(void* ws, int variant, bool diff, bool spec, bool fwd, int ntc) {
+  des::BatchLaunch bl;
+  bl.hdr = static_cast<const des::BatchHeader*>(ws);
+  bl.hdr_mut = static_cast<des::BatchHeader*>(ws);
+  bl.sets = reinterpret_cast<const des::BatchSet*>(static_cast<char*>(ws) + kHeaderBytes);
+  bl.ntc = ntc;
+  // table of up to 2048 texels, or the cross-chunk reduction scratch
+  const des::Tiling tl = tiling_for(ntc, des::kMaxSuperChunk, !fwd);
+  bl.lds = fwd ? tl.lds_fwd : tl.lds_bwd;
+  const int per_cu = des::batch_blocks_per_cu(variant, diff, spec, fwd, ntc, bl.lds);
+  bl.grid = unsigned(num_cus() * per_cu);
+  bl.reduce_grid = unsigned(num_cus() * 4);
+  return bl;
 }
 
 }  // namespace
@@ -407,6 +510,120 @@ DES_API int des_bruteforce_specularvisibility_shader_backward2(
                            true, enable_specular != 0, grad_normal, grad_view_dir, grad_albedo,
                            grad_specular_albedo, grad_roughness, grad_envmap, workspace,
                            workspace_bytes, stream, &pr);
+}
+
+DES_API int des_batch_plan(int32_t variant, const des_set* sets, int32_t n_sets,
+                           size_t* workspace_bytes, size_t* staging_bytes) {
+  if (variant < des::kDiffuse || variant > des::kVis)
+    return fail(DES_ERR_INVALID_ARGUMENT, "unknown variant %d", variant);
+  if (n_sets < 0 || (n_sets > 0 && sets == nullptr))
+    return fail(DES_ERR_INVALID_ARGUMENT, "sets must hold n_sets >= 0 descriptors");
+  for (int i = 0; i < n_sets; ++i)
+    if (int rc = check_common(sets[i].P, 3, sets[i].H, sets[i].W))
+      return fail(rc, "set %d: %s", i, g_last_error.c_str());
+  const BatchLayout L = batch_layout(sets, n_sets);
+  if (workspace_bytes) *workspace_bytes = L.total;
+  if (staging_bytes) *staging_bytes = L.table_bytes;
+  return ok();
+}
+
+DES_API int des_batch_prepare(int32_t variant, int enable_diffuse, int enable_specular,
+                              const des_set* sets, int32_t n_sets, int32_t ntc_forward,
+                              int32_t ntc_backward, int with_backward, void* staging,
+                              size_t staging_bytes, void* workspace, size_t workspace_bytes,
+                              void* stream) {
+  if (variant < des::kDiffuse || variant > des::kVis)
+    return fail(DES_ERR_INVALID_ARGUMENT, "unknown variant %d", variant);
+  if (n_sets < 0 || (n_sets > 0 && sets == nullptr))
+    return fail(DES_ERR_INVALID_ARGUMENT, "sets must hold n_sets >= 0 descriptors");
+  if (int rc = check_ntc(ntc_forward, 16, "ntc_forward")) return rc;
+  if (int rc = check_ntc(ntc_backward, 8, "ntc_backward")) return rc;
+  const bool diff = variant == des::kDiffuse || variant == des::kV2 || variant == des::kVis || enable_diffuse;
+  const bool spec = variant != des::kDiffuse && enable_specular;
+  for (int i = 0; i < n_sets; ++i)
+    if (int rc = check_set(variant, diff, spec, with_backward != 0, sets[i], i)) return rc;
+  const BatchLayout L = batch_layout(sets, n_sets);
+  if (staging == nullptr || staging_bytes < L.table_bytes)
+    return fail(DES_ERR_WORKSPACE_TOO_SMALL, "staging buffer of %zu bytes is smaller than the %zu required",
+                staging_bytes, L.table_bytes);
+  if (workspace == nullptr || workspace_bytes < L.total)
+    return fail(DES_ERR_WORKSPACE_TOO_SMALL, "workspace of %zu bytes is smaller than the %zu required",
+                workspace_bytes, L.total);
+  if ((reinterpret_cast<uintptr_t>(workspace) & 255) != 0)
+    return fail(DES_ERR_INVALID_ARGUMENT, "workspace must be 256-byte aligned");
+  char* st = static_cast<char*>(staging);
+  std::memset(st, 0, L.table_bytes);
+  auto* hdr = reinterpret_cast<des::BatchHeader*>(st);
+  auto* bs = reinterpret_cast<des::BatchSet*>(st + kHeaderBytes);
+  char* ws = static_cast<char*>(workspace);
+  int64_t tile0 = 0, r1 = 0, r2 = 0;
+  for (int i = 0; i < n_sets; ++i) {
+    const des_set& s = sets[i];
+    des::Problem pr = make_problem(s.P, s.H, s.W);
+    pr.N_face = s.N_face; pr.N_vertex = s.N_vertex;
+    pr.normal = s.normal; pr.view_dir = s.view_dir; pr.albedo = s.albedo;
+    pr.spec = s.specular_albedo; pr.rough = s.roughness; pr.envmap = s.envmap;
+    pr.rotmat = s.light2obj_rotmat;
+    pr.fvl = s.face_vertex_list; pr.tri = s.nearest_triangle_id; pr.bary = s.barycentric_coord;
+    pr.vis = reinterpret_cast<const uint32_t*>(s.visibility);
+    pr.sh = s.shading; pr.ds = s.diffuse_shading;
+    pr.g_sh = s.grad_shading; pr.g_ds = s.grad_diffuse_shading;
+    pr.gn = s.grad_normal; pr.gv = s.grad_view_dir; pr.ga = s.grad_albedo;
+    pr.gs = s.grad_specular_albedo; pr.gr = s.grad_roughness; pr.gE = s.grad_envmap;
+    pr.part = reinterpret_cast<float*>(ws + L.part_off[i]);
+    pr.part2 = reinterpret_cast<double*>(ws + L.part2_off[i]);
+    des::BatchSet& b = bs[i];
+    b.pr = pr;
+    b.tl_fwd = tiling_for(ntc_forward, pr.T, false);
+    b.tl_bwd = tiling_for(ntc_backward, pr.T, true);
+    const int64_t tiles = n_tiles(s.P);
+    b.tile0 = tile0;
+    b.ngroups = int32_t((tiles + des::kReduceRows - 1) / des::kReduceRows);
+    b.colblk = (3 * pr.T + 255) / 256;
+    b.r1_item0 = r1;
+    b.r2_item0 = r2;
+    tile0 += tiles;
+    r1 += int64_t(b.ngroups) * b.colblk;
+    r2 += b.colblk;
+  }
+  hdr->n_sets = n_sets;
+  hdr->variant = variant;
+  hdr->has_backward = with_backward ? 1 : 0;
+  hdr->total_tiles = tile0;
+  hdr->total_r1 = r1;
+  hdr->total_r2 = r2;
+  hipError_t e = hipMemcpyAsync(workspace, staging, L.table_bytes, hipMemcpyHostToDevice,
+                                static_cast<hipStream_t>(stream));
+  if (e != hipSuccess) return launch_error(e, "descriptor upload failed");
+  return ok();
+}
+
+DES_API int des_batch_forward(int32_t variant, int enable_diffuse, int enable_specular,
+                              int32_t ntc_forward, void* workspace, void* stream) {
+  if (variant < des::kDiffuse || variant > des::kVis)
+    return fail(DES_ERR_INVALID_ARGUMENT, "unknown variant %d", variant);
+  if (int rc = check_ntc(ntc_forward, 16, "ntc_forward")) return rc;
+  REQUIRE(workspace);
+  const bool diff = variant == des::kDiffuse || variant == des::kV2 || variant == des::kVis || enable_diffuse;
+  const bool spec = variant != des::kDiffuse && enable_specular;
+  const des::BatchLaunch bl = batch_launch(workspace, variant, diff, spec, true, ntc_forward);
+  hipError_t e = des::launch_batch_forward(variant, diff, spec, bl, static_cast<hipStream_t>(stream));
+  if (e != hipSuccess) return launch_error(e, "batched forward launch failed");
+  return ok();
+}
+
+DES_API int des_batch_backward(int32_t variant, int enable_diffuse, int enable_specular,
+                               int32_t ntc_backward, void* workspace, void* stream) {
+  if (variant < des::kDiffuse || variant > des::kVis)
+    return fail(DES_ERR_INVALID_ARGUMENT, "unknown variant %d", variant);
+  if (int rc = check_ntc(ntc_backward, 8, "ntc_backward")) return rc;
+  REQUIRE(workspace);
+  const bool diff = variant == des::kDiffuse || variant == des::kV2 || variant == des::kVis || enable_diffuse;
+  const bool spec = variant != des::kDiffuse && enable_specular;
+  const des::BatchLaunch bl = batch_launch(workspace, variant, diff, spec, false, ntc_backward);
+  hipError_t e = des::launch_batch_backward(variant, diff, spec, bl, static_cast<hipStream_t>(stream));
+  if (e != hipSuccess) return launch_error(e, "batched backward launch failed");
+  return ok();
 }
 
 }  // extern "C"

@@ -55,6 +55,46 @@ struct Tiling {
   int reduce_rows;  // partial rows summed per stage-1 group
 };
 
+// Batched launches: a device-resident descriptor table.
+struct BatchSet {
+  Problem pr;         // pointers + sizes of one point set (part/part2 inside the workspace)
+  Tiling tl_fwd;      // ntc equals the forward launch's ntc; cw/st follow this set's T
+  Tiling tl_bwd;      // same for the backward launch
+  int64_t tile0;      // first global tile of this set
+  int64_t r1_item0;   // first stage-1 reduce item (ngroups * colblk items per set)
+  int64_t r2_item0;   // first stage-2 reduce item (colblk items per set)
+  int32_t ngroups;    // ceil(tiles / reduce_rows)
+  int32_t colblk;     // ceil(3 T / 256)
+};
+
+struct BatchHeader {
+  int32_t n_sets;
+  int32_t variant;
+  int32_t has_backward;  // every set carries upstream gradients and gradient outputs
+  int32_t reserved;
+  int64_t total_tiles;
+  int64_t total_r1;
+  int64_t total_r2;
+  unsigned long long fwd_counter;  // work counters, zeroed by a kernel at each launch
+  unsigned long long bwd_counter;
+};
+
+struct BatchLaunch {
+  const BatchHeader* hdr;
+  BatchHeader* hdr_mut;
+  const BatchSet* sets;
+  int ntc;
+  size_t lds;
+  unsigned grid;
+  unsigned reduce_grid;
+};
+
+hipError_t launch_batch_forward(int variant, bool diff, bool spec, const BatchLaunch& bl,
+                                hipStream_t stream);
+hipError_t launch_batch_backward(int variant, bool diff, bool spec, const BatchLaunch& bl,
+                                 hipStream_t stream);
+int batch_blocks_per_cu(int variant, bool diff, bool spec, bool fwd, int ntc, size_t lds);
+
 constexpr int kForwardAcc = 6;
 constexpr int kBackwardAcc = 17;
 constexpr int kMaxSuperChunk = 2048;  // texels staged in LDS at once (64 KB)

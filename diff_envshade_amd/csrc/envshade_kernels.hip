@@ -264,13 +264,17 @@ __device__ __forceinline__ Lobe eval_lobe(const PointState<V, SPEC>& s, const fl
 
 // ---- forward -----------------------------------------------------------------
 
+// One 64-point tile of one problem; called by every wave of the workgroup.
+// Waves other than wave 0 return after the cross-chunk reduction; wave 0
+// writes the outputs.  No barrier follows the last one executed here, so a
+// caller reusing LDS must synchronise first.
 template <int V, bool DIFF, bool SPEC>
-__global__ __launch_bounds__(1024) void shade_forward_kernel(Problem pr, Tiling tl) {
+__device__ __forceinline__ void forward_tile(const Problem& pr, const Tiling& tl, int64_t tile) {
   extern __shared__ float4 smem[];
   constexpr bool DOUT = (V == kV2 || V == kVis);
   const int lane = threadIdx.x & 63;
   const int wv = threadIdx.x >> 6;
-  const int64_t p = (int64_t)blockIdx.x * 64 + lane;
+  const int64_t p = tile * 64 + lane;
   const bool valid = p < pr.P;
 
   PointState<V, SPEC> st;
@@ -365,12 +369,12 @@ __global__ __launch_bounds__(1024) void shade_forward_kernel(Problem pr, Tiling 
 //   (bs:62-71) expanded over h = (l+v) rs; their n- and v-directed parts are
 //   per-point constants times scalar sums (U, Q, Sb), applied once at the end.
 template <int V, bool DIFF, bool SPEC>
-__global__ __launch_bounds__(512) void shade_backward_kernel(Problem pr, Tiling tl) {
+__device__ __forceinline__ void backward_tile(const Problem& pr, const Tiling& tl, int64_t tile) {
   extern __shared__ float4 smem[];
   constexpr bool DOUT = (V == kV2 || V == kVis);
   const int lane = threadIdx.x & 63;
   const int wv = threadIdx.x >> 6;
-  const int64_t p = (int64_t)blockIdx.x * 64 + lane;
+  const int64_t p = tile * 64 + lane;
   const bool valid = p < pr.P;
 
   PointState<V, SPEC> st;
@@ -403,7 +407,7 @@ __global__ __launch_bounds__(512) void shade_backward_kernel(Problem pr, Tiling 
 
   float4* geo = smem;
   float4* rad = smem + tl.st;
-  float* part = pr.part + (int64_t)blockIdx.x * 3 * pr.T;
+  float* part = pr.part + tile * 3 * pr.T;
   // lanes (lane & 15) < 3 of row k store channel (lane & 15) of texel kRowTexel[k]
   const int row = lane >> 4;
   const int row_texel = (row == 1) ? 2 : ((row == 2) ? 1 : row);
@@ -547,14 +551,87 @@ __global__ __launch_bounds__(512) void shade_backward_kernel(Problem pr, Tiling 
   for (int c = 0; c < 3; ++c) pr.ga[3 * p + c] = DIFF ? gs[c] * acc[c] * kInvPi : 0.f;
 }
 
+// ---- single-problem launches: one workgroup per tile ---------------------------
+
+template <int V, bool DIFF, bool SPEC>
+__global__ __launch_bounds__(1024) void shade_forward_kernel(Problem pr, Tiling tl) {
+  forward_tile<V, DIFF, SPEC>(pr, tl, blockIdx.x);
+}
+
+template <int V, bool DIFF, bool SPEC>
+__global__ __launch_bounds__(512) void shade_backward_kernel(Problem pr, Tiling tl) {
+  backward_tile<V, DIFF, SPEC>(pr, tl, blockIdx.x);
+}
+
+// ---- batched launches: persistent workgroups pull tiles of many problems ------
+//
+// The descriptor table (BatchHeader + BatchSet[n]) lives in device memory, so a
+// captured graph of these launches can be replayed for any batch that fits the
+// workspace: sizes are read on the device, not baked into the launch.
+
+// Highest set index whose first item is <= g (sets with no items share the
+// next set's start, so the search lands on the set that owns g).
+template <typename F>
+__device__ __forceinline__ int find_set(int n, int64_t g, F first_item) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (first_item(mid) <= g) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ int64_t next_item(unsigned long long* counter) {
+  __shared__ int64_t s_item;
+  __syncthreads();
+  if (threadIdx.x == 0) s_item = (int64_t)atomicAdd(counter, 1ull);
+  __syncthreads();
+  return s_item;
+}
+
+template <int V, bool DIFF, bool SPEC>
+__global__ __launch_bounds__(1024) void batch_forward_kernel(const BatchHeader* __restrict__ hdr,
+                                                             const BatchSet* __restrict__ sets,
+                                                             unsigned long long* counter) {
+  const int64_t total = hdr->total_tiles;
+  const int n = hdr->n_sets;
+  for (;;) {
+    const int64_t g = next_item(counter);
+    if (g >= total) break;
+    const int s = __builtin_amdgcn_readfirstlane(
+        find_set(n, g, [&](int i) { return sets[i].tile0; }));
+    const Problem pr = sets[s].pr;
+    const Tiling tl = sets[s].tl_fwd;
+    forward_tile<V, DIFF, SPEC>(pr, tl, g - sets[s].tile0);
+  }
+}
+
+template <int V, bool DIFF, bool SPEC>
+__global__ __launch_bounds__(512) void batch_backward_kernel(const BatchHeader* __restrict__ hdr,
+                                                             const BatchSet* __restrict__ sets,
+                                                             unsigned long long* counter) {
+  if (!hdr->has_backward) return;
+  const int64_t total = hdr->total_tiles;
+  const int n = hdr->n_sets;
+  for (;;) {
+    const int64_t g = next_item(counter);
+    if (g >= total) break;
+    const int s = __builtin_amdgcn_readfirstlane(
+        find_set(n, g, [&](int i) { return sets[i].tile0; }));
+    const Problem pr = sets[s].pr;
+    const Tiling tl = sets[s].tl_bwd;
+    backward_tile<V, DIFF, SPEC>(pr, tl, g - sets[s].tile0);
+  }
+}
+
 // ---- env-map gradient: fixed-order FP64 reduction of the per-tile partials -----
 
-// stage 1: rows [g*R, min((g+1)*R, nrows)) of part[nrows][ncol] -> part2[g][ncol]
-__global__ __launch_bounds__(256) void reduce_partials_kernel(const float* __restrict__ part,
-                                                              double* __restrict__ part2,
-                                                              int64_t nrows, int ncol, int R) {
-  const int col = blockIdx.x * 256 + threadIdx.x;
-  const int64_t g = blockIdx.y;
+// stage 1: rows [g*R, min((g+1)*R, nrows)) of part[nrows][ncol] -> part2[g][ncol],
+// columns [cb*256, cb*256+256)
+__device__ __forceinline__ void reduce_rows_item(const float* __restrict__ part,
+                                                 double* __restrict__ part2, int64_t nrows,
+                                                 int ncol, int R, int64_t g, int cb) {
+  const int col = cb * 256 + threadIdx.x;
   if (col >= ncol) return;
   const int64_t r0 = g * R;
   const int64_t r1 = min<int64_t>(r0 + R, nrows);
@@ -575,11 +652,11 @@ __global__ __launch_bounds__(256) void reduce_partials_kernel(const float* __res
 }
 
 // stage 2: gE[c][t] = dOmega_t * sum_g part2[g][c*T + t], rounded once to fp32
-__global__ __launch_bounds__(256) void finish_envgrad_kernel(const double* __restrict__ part2,
-                                                             float* __restrict__ gE, int ngroups,
-                                                             int H, int W) {
+__device__ __forceinline__ void finish_item(const double* __restrict__ part2,
+                                            float* __restrict__ gE, int ngroups, int H, int W,
+                                            int cb) {
   const int T = H * W;
-  const int col = blockIdx.x * 256 + threadIdx.x;
+  const int col = cb * 256 + threadIdx.x;
   if (col >= 3 * T) return;
   const int t = col % T;
   const int iy = H - 1 - t / W;
@@ -588,6 +665,51 @@ __global__ __launch_bounds__(256) void finish_envgrad_kernel(const double* __res
   const double dw = 4.0 * kPiD * sinpi(0.5 / (double)H) / (double)W *
                     sinpi(((double)iy + 0.5) / (double)H);
   gE[col] = (float)(s * dw);
+}
+
+__global__ __launch_bounds__(256) void reduce_partials_kernel(const float* __restrict__ part,
+                                                              double* __restrict__ part2,
+                                                              int64_t nrows, int ncol, int R) {
+  reduce_rows_item(part, part2, nrows, ncol, R, blockIdx.y, blockIdx.x);
+}
+
+__global__ __launch_bounds__(256) void finish_envgrad_kernel(const double* __restrict__ part2,
+                                                             float* __restrict__ gE, int ngroups,
+                                                             int H, int W) {
+  finish_item(part2, gE, ngroups, H, W, blockIdx.x);
+}
+
+__global__ __launch_bounds__(256) void batch_reduce_partials_kernel(
+    const BatchHeader* __restrict__ hdr, const BatchSet* __restrict__ sets) {
+  if (!hdr->has_backward) return;
+  const int64_t total = hdr->total_r1;
+  const int n = hdr->n_sets;
+  for (int64_t item = blockIdx.x; item < total; item += gridDim.x) {
+    const int s = find_set(n, item, [&](int i) { return sets[i].r1_item0; });
+    const int64_t local = item - sets[s].r1_item0;
+    const int cb = (int)(local % sets[s].colblk);
+    const int64_t g = local / sets[s].colblk;
+    const Problem& pr = sets[s].pr;
+    reduce_rows_item(pr.part, pr.part2, (pr.P + 63) / 64, 3 * pr.T, sets[s].tl_bwd.reduce_rows, g,
+                     cb);
+  }
+}
+
+__global__ __launch_bounds__(256) void batch_finish_envgrad_kernel(
+    const BatchHeader* __restrict__ hdr, const BatchSet* __restrict__ sets) {
+  if (!hdr->has_backward) return;
+  const int64_t total = hdr->total_r2;
+  const int n = hdr->n_sets;
+  for (int64_t item = blockIdx.x; item < total; item += gridDim.x) {
+    const int s = find_set(n, item, [&](int i) { return sets[i].r2_item0; });
+    const int cb = (int)(item - sets[s].r2_item0);
+    const Problem& pr = sets[s].pr;
+    finish_item(pr.part2, pr.gE, sets[s].ngroups, pr.H, pr.W, cb);
+  }
+}
+
+__global__ __launch_bounds__(64) void zero_counters_kernel(unsigned long long* c, int n) {
+  if ((int)threadIdx.x < n) c[threadIdx.x] = 0ull;
 }
 
 __global__ __launch_bounds__(256) void fill_zero_kernel(float* __restrict__ x, int64_t n) {
@@ -684,6 +806,91 @@ hipError_t launch_fill_zero(float* x, int64_t n, hipStream_t stream) {
   hipLaunchKernelGGL(fill_zero_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, x,
                      n);
   return hipGetLastError();
+}
+
+
+namespace {
+
+template <int V, bool DIFF, bool SPEC>
+hipError_t launch_bfwd(const BatchLaunch& bl, hipStream_t stream) {
+  hipLaunchKernelGGL((batch_forward_kernel<V, DIFF, SPEC>), dim3(bl.grid), dim3(64 * bl.ntc),
+                     bl.lds, stream, bl.hdr, bl.sets, &bl.hdr_mut->fwd_counter);
+  return hipGetLastError();
+}
+
+template <int V, bool DIFF, bool SPEC>
+hipError_t launch_bbwd(const BatchLaunch& bl, hipStream_t stream) {
+  hipLaunchKernelGGL((batch_backward_kernel<V, DIFF, SPEC>), dim3(bl.grid), dim3(64 * bl.ntc),
+                     bl.lds, stream, bl.hdr, bl.sets, &bl.hdr_mut->bwd_counter);
+  return hipGetLastError();
+}
+
+template <bool FWD, int V, bool DIFF, bool SPEC>
+hipError_t launch_b(const BatchLaunch& bl, hipStream_t stream) {
+  return FWD ? launch_bfwd<V, DIFF, SPEC>(bl, stream) : launch_bbwd<V, DIFF, SPEC>(bl, stream);
+}
+
+template <bool FWD>
+hipError_t dispatch_batch(int variant, bool diff, bool spec, const BatchLaunch& bl,
+                          hipStream_t stream) {
+  switch (variant) {
+    case kDiffuse: return launch_b<FWD, kDiffuse, true, false>(bl, stream);
+    case kSpecular:
+      if (diff && spec) return launch_b<FWD, kSpecular, true, true>(bl, stream);
+      if (diff) return launch_b<FWD, kSpecular, true, false>(bl, stream);
+      if (spec) return launch_b<FWD, kSpecular, false, true>(bl, stream);
+      return launch_b<FWD, kSpecular, false, false>(bl, stream);
+    case kClamp:
+      if (diff && spec) return launch_b<FWD, kClamp, true, true>(bl, stream);
+      if (diff) return launch_b<FWD, kClamp, true, false>(bl, stream);
+      if (spec) return launch_b<FWD, kClamp, false, true>(bl, stream);
+      return launch_b<FWD, kClamp, false, false>(bl, stream);
+    case kV2:
+      return spec ? launch_b<FWD, kV2, true, true>(bl, stream) : launch_b<FWD, kV2, true, false>(bl, stream);
+    case kVis:
+      return spec ? launch_b<FWD, kVis, true, true>(bl, stream) : launch_b<FWD, kVis, true, false>(bl, stream);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+}  // namespace
+
+hipError_t launch_batch_forward(int variant, bool diff, bool spec, const BatchLaunch& bl,
+                                hipStream_t stream) {
+  hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(64), 0, stream, &bl.hdr_mut->fwd_counter, 1);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  return dispatch_batch<true>(variant, diff, spec, bl, stream);
+}
+
+hipError_t launch_batch_backward(int variant, bool diff, bool spec, const BatchLaunch& bl,
+                                 hipStream_t stream) {
+  hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(64), 0, stream, &bl.hdr_mut->bwd_counter, 1);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  e = dispatch_batch<false>(variant, diff, spec, bl, stream);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(batch_reduce_partials_kernel, dim3(bl.reduce_grid), dim3(256), 0, stream,
+                     bl.hdr, bl.sets);
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(batch_finish_envgrad_kernel, dim3(bl.reduce_grid), dim3(256), 0, stream,
+                     bl.hdr, bl.sets);
+  return hipGetLastError();
+}
+
+int batch_blocks_per_cu(int variant, bool diff, bool spec, bool fwd, int ntc, size_t lds) {
+  // all instantiations of one direction share launch bounds; use the heaviest one
+  int n = 0;
+  hipError_t e;
+  if (fwd)
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, batch_forward_kernel<kVis, true, true>,
+                                                     64 * ntc, lds);
+  else
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, batch_backward_kernel<kVis, true, true>,
+                                                     64 * ntc, lds);
+  (void)variant; (void)diff; (void)spec;
+  return (e == hipSuccess && n > 0) ? n : 1;
 }
 
 }  // namespace des

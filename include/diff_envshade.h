@@ -180,6 +180,78 @@ DES_API int des_bruteforce_specularvisibility_shader_backward2(
  * choice).  Process-wide; results do not depend on it beyond fp32 rounding. */
 DES_API int des_set_tiling_override(int32_t ntc_forward, int32_t ntc_backward);
 
+/* ---- batched launches (no counterpart in the reference) --------------------
+ * Many point sets, each with its own env map and rotation, shaded by one
+ * persistent launch per direction.  All sets of a batch use one variant and
+ * one pair of flags.  Per-set results equal those of the single-set entry
+ * points with the same launch shape (des_set_tiling_override), so a set's
+ * results do not depend on which other sets share its batch.
+ *
+ *   1. des_batch_plan     -> workspace (device) and staging (host) sizes
+ *   2. des_batch_prepare  -> validates the sets, writes the descriptor table
+ *                            into `staging` and enqueues its upload into the
+ *                            workspace on `stream` (keep `staging` alive, and
+ *                            preferably pinned, until the copy has run)
+ *   3. des_batch_forward / des_batch_backward -> persistent launches that read
+ *      every size from the workspace; they can be captured once in a HIP graph
+ *      and replayed for any later batch prepared into the same workspace.
+ */
+typedef struct des_set {
+  int64_t P;
+  int32_t H, W;
+  int32_t N_face, N_vertex, N_slot; /* visibility variant only */
+  int32_t reserved;
+  const float* normal;
+  const float* view_dir;
+  const float* albedo;
+  const float* specular_albedo;
+  const float* roughness;
+  const float* envmap;
+  const float* light2obj_rotmat;
+  const int32_t* face_vertex_list;
+  const int64_t* nearest_triangle_id;
+  const float* barycentric_coord;
+  const int32_t* visibility;
+  float* shading;
+  float* diffuse_shading;           /* v2 / visibility variants */
+  const float* grad_shading;        /* backward only (NULL for forward-only batches) */
+  const float* grad_diffuse_shading;
+  float* grad_normal;
+  float* grad_view_dir;
+  float* grad_albedo;
+  float* grad_specular_albedo;
+  float* grad_roughness;
+  float* grad_envmap;
+} des_set;
+
+/* Variant ids of the batched entry points. */
+enum {
+  DES_VARIANT_DIFFUSE = 0,
+  DES_VARIANT_SPECULAR = 1,
+  DES_VARIANT_CLAMP = 2,
+  DES_VARIANT_V2 = 3,
+  DES_VARIANT_VISIBILITY = 4
+};
+
+DES_API int des_batch_plan(int32_t variant, const des_set* sets, int32_t n_sets,
+                           size_t* workspace_bytes, size_t* staging_bytes);
+
+/* ntc_forward / ntc_backward: waves per 64-point tile (powers of two, <= 16 / <= 8),
+ * usually from the autotuner; they must match the later launches.  with_backward:
+ * the sets carry upstream gradients and gradient outputs. */
+DES_API int des_batch_prepare(int32_t variant, int enable_diffuse, int enable_specular,
+                              const des_set* sets, int32_t n_sets, int32_t ntc_forward,
+                              int32_t ntc_backward, int with_backward, void* staging,
+                              size_t staging_bytes, void* workspace, size_t workspace_bytes,
+                              void* stream);
+
+DES_API int des_batch_forward(int32_t variant, int enable_diffuse, int enable_specular,
+                              int32_t ntc_forward, void* workspace, void* stream);
+
+/* Does nothing on the device when the batch was prepared without with_backward. */
+DES_API int des_batch_backward(int32_t variant, int enable_diffuse, int enable_specular,
+                               int32_t ntc_backward, void* workspace, void* stream);
+
 /* Timing hook for benchmarks (no counterpart in the reference): when set,
  * the calling thread's next *_backward* call records hipEvent_t `start` right
  * before and `stop` right after its main shading kernel, on the call's stream,

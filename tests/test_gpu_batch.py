@@ -1,0 +1,159 @@
+"""Batched persistent launches (GPU): every set of a batch matches the oracle,
+matches the single-set binding with the same launch shape bit for bit, and a
+captured graph replays correctly for later batches."""
+import numpy as np
+import pytest
+import torch
+
+from _helpers import TOL, compare, gpu_inputs, gpu_run, oracle_run, problem
+
+pytestmark = pytest.mark.gpu
+
+
+def _sets(specs, seed=0):
+    """specs: list of (n, H, W) -> (numpy problems, device dicts)."""
+    nps, devs = [], []
+    for i, (n, H, W) in enumerate(specs):
+        d = problem(seed=seed + i, n=n, H=H, W=W, vis="identity")
+        nps.append(d)
+        devs.append(gpu_inputs(d))
+    return nps, devs
+
+
+def _np(outs, variant="vis"):
+    fwd = [outs["shading"], outs["diffuse_shading"]]
+    bwd = [outs[k] for k in ("grad_normal", "grad_view_dir", "grad_albedo",
+                             "grad_specular_albedo", "grad_roughness", "grad_envmap")]
+    return [x.cpu().numpy() for x in fwd], [x.cpu().numpy() for x in bwd]
+
+
+SPECS = [(300, 16, 32), (5000, 32, 64), (1, 4, 8), (0, 16, 32), (2049, 16, 32), (777, 8, 16)]
+
+
+@pytest.mark.parametrize("ntc_f,ntc_b", [(8, 8), (4, 2), (16, 1)])
+def test_batch_matches_oracle_and_single_set(ntc_f, ntc_b):
+    from diff_envshade_amd import _lib
+    from diff_envshade_amd.batch import ShadeBatch
+    nps, devs = _sets(SPECS, seed=20)
+    b = ShadeBatch("vis", enable_specular=True, ntc_forward=ntc_f, ntc_backward=ntc_b)
+    outs = b.prepare(devs, with_backward=True)
+    b.forward()
+    b.backward()
+    torch.cuda.synchronize()
+    try:
+        _lib.set_tiling_override(ntc_f, ntc_b)
+        for d, o in zip(nps, outs):
+            got = _np(o)
+            if d["normal"].shape[0] > 0:
+                compare("vis", got, oracle_run("vis", d))
+            single = gpu_run("vis", d)
+            for a, s in zip(got[0] + got[1], single[0] + single[1]):
+                assert np.array_equal(a.reshape(s.shape), s)
+    finally:
+        _lib.set_tiling_override(0, 0)
+
+
+def test_batch_results_independent_of_batch_mates():
+    from diff_envshade_amd.batch import ShadeBatch
+    nps, devs = _sets(SPECS, seed=30)
+    b = ShadeBatch("vis")
+    b.prepare(devs)
+    b.forward()
+    b.backward()
+    torch.cuda.synchronize()
+    together = [_np(o) for o in b.outputs]
+    for i, dev in enumerate(devs):
+        alone = ShadeBatch("vis")
+        o = alone.prepare([dev])
+        alone.forward()
+        alone.backward()
+        torch.cuda.synchronize()
+        a = _np(o[0])
+        for x, y in zip(a[0] + a[1], together[i][0] + together[i][1]):
+            assert np.array_equal(x, y)
+
+
+def test_graph_capture_and_replay_new_batches():
+    from diff_envshade_amd.batch import ShadeBatch
+    nps, devs = _sets([(4000, 32, 64), (3000, 16, 32), (100, 16, 32)], seed=40)
+    b = ShadeBatch("vis")
+    b.prepare(devs)
+    b.capture()
+    for rep, specs in enumerate([[(2500, 16, 32), (700, 32, 64)], [(4000, 32, 64)],
+                                 [(10, 4, 8), (3000, 16, 32), (2000, 16, 32)]]):
+        nps2, devs2 = _sets(specs, seed=50 + 10 * rep)
+        outs = b.prepare(devs2)
+        b.replay()
+        torch.cuda.synchronize()
+        graph_res = [_np(o) for o in outs]
+        # eager work between replays, then an eager run of the same batch
+        e = ShadeBatch("vis")
+        eo = e.prepare(devs2)
+        e.forward()
+        e.backward()
+        torch.cuda.synchronize()
+        for g, x, d in zip(graph_res, eo, nps2):
+            ex = _np(x)
+            for a, c in zip(g[0] + g[1], ex[0] + ex[1]):
+                assert np.array_equal(a, c)
+            compare("vis", g, oracle_run("vis", d))
+
+
+def test_forward_only_batch():
+    from diff_envshade_amd.batch import ShadeBatch
+    nps, devs = _sets([(1000, 16, 32), (64, 8, 8)], seed=60)
+    for d in devs:
+        d.pop("grad_shading")
+        d.pop("grad_diffuse_shading")
+    b = ShadeBatch("vis")
+    outs = b.prepare(devs, with_backward=False)
+    b.forward()
+    b.backward()  # no-op on a forward-only batch
+    torch.cuda.synchronize()
+    for d, o in zip(nps, outs):
+        sh, ds = oracle_run("vis", d)[0]
+        from _helpers import relerr
+        assert relerr(o["shading"].cpu().numpy(), sh) <= TOL
+        assert relerr(o["diffuse_shading"].cpu().numpy(), ds) <= TOL
+
+
+def test_autotuner_picks_valid_shapes():
+    from diff_envshade_amd.autotune import tune_batch
+    from diff_envshade_amd.batch import ShadeBatch
+    nps, devs = _sets([(20000, 16, 32), (9000, 32, 64)], seed=70)
+    b = ShadeBatch("vis")
+    f, bw = tune_batch(b, devs, reps=2, force=True)
+    assert f in (1, 2, 4, 8, 16) and bw in (1, 2, 4, 8)
+    b.forward()
+    b.backward()
+    torch.cuda.synchronize()
+    for d, o in zip(nps, b.outputs):
+        compare("vis", _np(o), oracle_run("vis", d))
+
+
+@pytest.mark.parametrize("variant", ["diffuse", "specular", "clamp", "v2"])
+def test_batch_other_variants(variant):
+    from diff_envshade_amd.batch import ShadeBatch
+    nps, devs = [], []
+    for i, (n, H, W) in enumerate([(500, 8, 16), (130, 16, 32)]):
+        d = problem(seed=80 + i, n=n, H=H, W=W)
+        if variant not in ("v2",):
+            d.pop("grad_diffuse_shading")
+        nps.append(d)
+        devs.append(gpu_inputs(d))
+    b = ShadeBatch(variant, enable_diffuse=True, enable_specular=True)
+    outs = b.prepare(devs)
+    b.forward()
+    b.backward()
+    torch.cuda.synchronize()
+    for d, o in zip(nps, outs):
+        fwd = [o["shading"].cpu().numpy()]
+        if variant == "v2":
+            fwd.append(o["diffuse_shading"].cpu().numpy())
+        if variant == "diffuse":
+            names = ("grad_normal", "grad_albedo", "grad_envmap")
+        else:
+            names = ("grad_normal", "grad_view_dir", "grad_albedo", "grad_specular_albedo",
+                     "grad_roughness", "grad_envmap")
+        bwd = [o[k].cpu().numpy() for k in names]
+        compare(variant, (fwd, bwd), oracle_run(variant, d))
