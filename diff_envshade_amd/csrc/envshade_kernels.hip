@@ -23,20 +23,13 @@
 #include <stdint.h>
 
 #include "envshade_internal.h"
+#include "shade_math.h"
 
 namespace des {
 
 namespace {
 
-constexpr float kF0 = 0.04f;      // Schlick R0 (bs:30)
-constexpr float kThres = 1e-9f;   // NDF denominator floor (bs:31)
-constexpr float kInvPi = 0.318309886183790671538f;
-constexpr float kPi = 3.14159265358979323846f;
-constexpr double kPiD = 3.14159265358979323846;
-
-__device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
-__device__ __forceinline__ float rsq(float x) { return __builtin_amdgcn_rsqf(x); }
-__device__ __forceinline__ float sat(float x) { return __builtin_amdgcn_fmed3f(x, 0.f, 1.f); }
+using namespace math;
 
 // ---- cross-lane helpers (wave64) -------------------------------------------------
 
@@ -80,31 +73,20 @@ __device__ __forceinline__ float reduce4(float q0, float q1, float q2, float q3)
 //   l = M d with M row-major.
 __device__ void build_table(const Problem& pr, int t0, int n, int n_alloc, float4* geo,
                             float4* rad) {
-  const int H = pr.H, W = pr.W, T = pr.T;
   double M[9];
 #pragma unroll
   for (int i = 0; i < 9; ++i) M[i] = (double)pr.rotmat[i];
-  const double kdw = 4.0 * kPiD * sinpi(0.5 / (double)H) / (double)W;
+  const double kdw = solid_angle_scale(pr.H, pr.W);
+  const int T = pr.T;
   for (int i = threadIdx.x; i < n_alloc; i += blockDim.x) {
     float4 g = make_float4(0.f, 0.f, 0.f, 0.f);
     float4 e = make_float4(0.f, 0.f, 0.f, 0.f);
     if (i < n) {
       const int t = t0 + i;
-      const int r = t / W;
-      const int x = t - r * W;
-      const int iy = H - 1 - r;
-      double sth, cth, sph, cph;
-      sincospi(((double)iy + 0.5) / (double)H - 0.5, &sth, &cth);
-      sincospi((2.0 * (double)x + 1.0) / (double)W, &sph, &cph);
-      const double dx = cth * cph, dy = sth, dz = cth * sph;
-      const float lx = (float)(M[0] * dx + M[1] * dy + M[2] * dz);
-      const float ly = (float)(M[3] * dx + M[4] * dy + M[5] * dz);
-      const float lz = (float)(M[6] * dx + M[7] * dy + M[8] * dz);
-      const float ll = (float)((double)lx * lx + (double)ly * ly + (double)lz * lz);
-      const double dw = cth * kdw;
-      g = make_float4(lx, ly, lz, ll);
-      e = make_float4((float)(pr.envmap[t] * dw), (float)(pr.envmap[T + t] * dw),
-                      (float)(pr.envmap[2 * T + t] * dw), (float)dw);
+      const TexelGeo tg = texel_geometry(t, pr.H, pr.W, M, kdw);
+      g = make_float4(tg.lx, tg.ly, tg.lz, tg.ll);
+      e = make_float4((float)(pr.envmap[t] * tg.dw), (float)(pr.envmap[T + t] * tg.dw),
+                      (float)(pr.envmap[2 * T + t] * tg.dw), (float)tg.dw);
     }
     geo[i] = g;
     rad[i] = e;
@@ -211,57 +193,6 @@ __device__ __forceinline__ float vis_weight(const PointState<V, SPEC>& s, const 
   return fmaf(b2, s.c2, fmaf(b1, s.c1, b0 * s.c0));
 }
 
-// ---- the per-pair lobe -------------------------------------------------------
-
-// Quantities of one (point, texel) pair shared by forward and backward.
-struct Lobe {
-  float cs;    // l.n
-  float nl;    // max(cs, 0)
-  float rs;    // 1/|l+v|
-  float ch;    // l.h
-  float nh;    // n.h
-  float om4;   // (1 - sat(ch))^4
-  float F, G, D;
-  float ia;    // 1/(nl(1-k)+k)
-  float it2;   // 1/max(pi tD^2, thres)
-  float tD;    // (n.h)^2 (alpha^2-1) + 1
-  float tmp;   // F G D / 4, or 0 where the reference skips the lobe (tmp <= 0 or NaN)
-  bool spec_on;
-};
-
-template <int V, bool SPEC>
-__device__ __forceinline__ Lobe eval_lobe(const PointState<V, SPEC>& s, const float4& g) {
-  Lobe o;
-  o.cs = fmaf(g.z, s.nz, fmaf(g.y, s.ny, g.x * s.nx));
-  o.nl = fmaxf(o.cs, 0.f);
-  if (SPEC) {
-    const float lv = fmaf(g.z, s.vz, fmaf(g.y, s.vy, g.x * s.vx));
-    // |l+v|^2 = |l|^2 + 2 l.v + |v|^2;  h = (l+v)/|l+v|
-    const float hb2 = fmaf(2.f, lv, g.w + s.vv);
-    // floor keeps (n.h)^2 finite when v = -l exactly; such pairs are masked below
-    o.rs = rsq(fmaxf(hb2, 1e-20f));
-    o.ch = (g.w + lv) * o.rs;
-    o.nh = (o.cs + s.nv) * o.rs;
-    const float om = 1.f - sat(o.ch);
-    const float om2 = om * om;
-    o.om4 = om2 * om2;
-    o.F = fmaf(o.om4 * om, 1.f - kF0, kF0);
-    o.ia = rcp(fmaf(o.nl, s.omk, s.k));
-    o.G = o.nl * o.ia * s.ib;
-    o.tD = fmaf(o.nh * o.nh, s.am1, 1.f);
-    o.it2 = rcp(fmaxf(kPi * o.tD * o.tD, kThres));
-    o.D = s.a2 * o.it2;
-    const float tmp = o.F * o.G * o.D * 0.25f;
-    // the reference skips the lobe unless tmp > 0; |l+v| = 0 gives NaN there
-    o.spec_on = (tmp > 0.f) && (hb2 > 0.f);
-    o.tmp = o.spec_on ? tmp : 0.f;
-  } else {
-    o.spec_on = false;
-    o.tmp = 0.f;
-  }
-  return o;
-}
-
 // ---- forward -----------------------------------------------------------------
 
 // One 64-point tile of one problem; called by every wave of the workgroup.
@@ -305,7 +236,7 @@ __device__ __forceinline__ void forward_tile(const Problem& pr, const Tiling& tl
         const float4 g = geo[sl + b];
         const float4 e = rad[sl + b];
         const float vis = vis_weight<V, SPEC>(st, w, b, single, csum);
-        const Lobe lb = eval_lobe<V, SPEC>(st, g);
+        const Lobe lb = eval_lobe<SPEC>(st, g);
         if (DIFF) {
           const float x = lb.nl * vis;
           ds0 = fmaf(e.x, x, ds0);
@@ -431,7 +362,7 @@ __device__ __forceinline__ void backward_tile(const Problem& pr, const Tiling& t
           const float4 g = geo[sl + b];
           const float4 e = rad[sl + b];
           const float vis = vis_weight<V, SPEC>(st, w, b, single, csum);
-          const Lobe lb = eval_lobe<V, SPEC>(st, g);
+          const Lobe lb = eval_lobe<SPEC>(st, g);
           float coef_n = 0.f;
           float x = 0.f, y = 0.f;
           if (DIFF) {
@@ -662,8 +593,7 @@ __device__ __forceinline__ void finish_item(const double* __restrict__ part2,
   const int iy = H - 1 - t / W;
   double s = 0.0;
   for (int g = 0; g < ngroups; ++g) s += part2[(int64_t)g * 3 * T + col];
-  const double dw = 4.0 * kPiD * sinpi(0.5 / (double)H) / (double)W *
-                    sinpi(((double)iy + 0.5) / (double)H);
+  const double dw = solid_angle_scale(H, W) * sinpi(((double)iy + 0.5) / (double)H);
   gE[col] = (float)(s * dw);
 }
 
