@@ -102,6 +102,9 @@ def main() -> int:
     ev = lambda: torch.cuda.Event(enable_timing=True)  # noqa: E731
     step_ev = [(ev(), ev()) for _ in range(K)]
     kern_ev = [(ev(), ev()) for _ in range(K)]
+    for a, b in kern_ev:  # torch creates the HIP events on first record
+        a.record()
+        b.record()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for i in range(K):

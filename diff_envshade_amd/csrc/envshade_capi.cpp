@@ -15,13 +15,9 @@
 #include "../../include/diff_envshade.h"
 #include "envshade_internal.h"
 
-namespace {
+namespace des_capi {
 
 thread_local std::string g_last_error;
-thread_local hipEvent_t g_bwd_ev_start = nullptr;
-thread_local hipEvent_t g_bwd_ev_stop = nullptr;
-std::atomic<int> g_ntc_fwd_override{0};
-std::atomic<int> g_ntc_bwd_override{0};
 
 int fail(int code, const char* fmt, ...) {
   char buf[512];
@@ -37,6 +33,19 @@ int ok() {
   g_last_error.clear();
   return DES_OK;
 }
+
+}  // namespace des_capi
+
+namespace {
+
+using des_capi::fail;
+using des_capi::g_last_error;
+using des_capi::ok;
+
+thread_local hipEvent_t g_bwd_ev_start = nullptr;
+thread_local hipEvent_t g_bwd_ev_stop = nullptr;
+std::atomic<int> g_ntc_fwd_override{0};
+std::atomic<int> g_ntc_bwd_override{0};
 
 constexpr int kTileP = 64;
 constexpr int64_t kMaxTexels = int64_t(1) << 24;

@@ -96,7 +96,7 @@ hipError_t launch_batch_backward(int variant, bool diff, bool spec, const BatchL
 int batch_blocks_per_cu(int variant, bool diff, bool spec, bool fwd, int ntc, size_t lds);
 
 constexpr int kForwardAcc = 6;
-constexpr int kBackwardAcc = 17;
+constexpr int kBackwardAcc = 22;
 constexpr int kMaxSuperChunk = 2048;  // texels staged in LDS at once (64 KB)
 constexpr int kReduceRows = 32;
 

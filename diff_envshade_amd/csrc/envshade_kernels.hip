@@ -296,9 +296,10 @@ __device__ __forceinline__ void forward_tile(const Problem& pr, const Tiling& tl
 //   grad_spec_c     = gs_c * sum E'_c y
 //   gE_c(t)         = dOmega_t * sum_p (A_c x + B_c y)
 //   l-coefficients of grad_normal / grad_view_dir come from the diffuse term,
-//   the G1 VJP (bs:173-201), the NDF VJP (bs:212-227) and the normalize VJP
-//   (bs:62-71) expanded over h = (l+v) rs; their n- and v-directed parts are
-//   per-point constants times scalar sums (U, Q, Sb), applied once at the end.
+//   the G1 VJP (bs:173-201) and the normalize VJP (bs:62-71); the half-vector
+//   terms of the NDF and normalize VJPs are accumulated along l + v; the
+//   n- and v-directed parts are per-point constants times scalar sums (U, Sb),
+//   applied once at the end.
 template <int V, bool DIFF, bool SPEC>
 __device__ __forceinline__ void backward_tile(const Problem& pr, const Tiling& tl, int64_t tile) {
   extern __shared__ float4 smem[];
@@ -334,7 +335,9 @@ __device__ __forceinline__ void backward_tile(const Problem& pr, const Tiling& t
   float sp0 = 0.f, sp1 = 0.f, sp2 = 0.f;
   float lnx = 0.f, lny = 0.f, lnz = 0.f;  // sum coef_n l
   float lvx = 0.f, lvy = 0.f, lvz = 0.f;  // sum coef_v l
-  float U = 0.f, Q = 0.f, Sb = 0.f, ga2 = 0.f, gk = 0.f;
+  float hnx = 0.f, hny = 0.f, hnz = 0.f;  // sum rs gdnh (l + v)
+  float hvx = 0.f, hvy = 0.f, hvz = 0.f;  // sum q (l + v)
+  float U = 0.f, Sb = 0.f, ga2 = 0.f, gk = 0.f;
 
   float4* geo = smem;
   float4* rad = smem + tl.st;
@@ -401,14 +404,23 @@ __device__ __forceinline__ void backward_tile(const Problem& pr, const Tiling& t
             const float nh2 = lb.nh * lb.nh;
             ga2 = fmaf(nh2, g_t, ga2);
             const float gdnh = 2.f * g_t * st.am1 * lb.nh;
-            // h = (l+v) rs; grad_h = gch l + gdnh n; normalize VJP:
-            //   rs (grad_h - h (h.grad_h)) = rs gch l + rs gdnh n - q (l + v)
+            // h = hb rs with hb = l + v; grad_h = gch l + gdnh n.
+            //   NDF term of grad_normal:   gdnh h = (rs gdnh) hb
+            //   normalize VJP (bs:62-71):  rs (grad_h - h (h.grad_h))
+            //                            = rs gch l + rs gdnh n - q hb
+            // hb is accumulated as a vector: splitting it into l and v parts
+            // would cancel by ~1/|l+v| when l and v are nearly opposite.
             const float rg = lb.rs * gdnh;
             const float q2 = lb.rs * lb.rs * fmaf(gch, lb.ch, gdnh * lb.nh);
             U += rg;
-            Q += q2;
-            coef_n += gdl + rg;
-            const float coef_v = fmaf(lb.rs, gch, -q2);
+            coef_n += gdl;
+            hnx = fmaf(rg, lb.hx, hnx);
+            hny = fmaf(rg, lb.hy, hny);
+            hnz = fmaf(rg, lb.hz, hnz);
+            hvx = fmaf(q2, lb.hx, hvx);
+            hvy = fmaf(q2, lb.hy, hvy);
+            hvz = fmaf(q2, lb.hz, hvz);
+            const float coef_v = lb.rs * gch;
             lvx = fmaf(coef_v, g.x, lvx);
             lvy = fmaf(coef_v, g.y, lvy);
             lvz = fmaf(coef_v, g.z, lvz);
@@ -433,9 +445,9 @@ __device__ __forceinline__ void backward_tile(const Problem& pr, const Tiling& t
   }
 
   // fixed-order reduction of the per-point sums over the NTC texel chunks
-  constexpr int NACC = 17;
-  float acc[NACC] = {ds0, ds1, ds2, sp0, sp1, sp2, lnx, lny, lnz,
-                     lvx, lvy, lvz, U,   Q,   Sb,  ga2, gk};
+  constexpr int NACC = kBackwardAcc;
+  float acc[NACC] = {ds0, ds1, ds2, sp0, sp1, sp2, lnx, lny, lnz, lvx, lvy,
+                     lvz, hnx, hny, hnz, hvx, hvy, hvz, U,   Sb,  ga2, gk};
   if (tl.ntc > 1) {
     __syncthreads();
     float* red = reinterpret_cast<float*>(smem);
@@ -451,19 +463,19 @@ __device__ __forceinline__ void backward_tile(const Problem& pr, const Tiling& t
     }
   }
   if (!valid) return;
-  // gn = sum coef_n l + v (U + [n.v>0](1-k) Sb)
-  // gv = sum coef_v l + n (U + [n.v>0](1-k) Sb) - v Q
+  // gn = sum coef_n l + sum rs gdnh (l+v) + v [n.v>0](1-k) Sb
+  // gv = sum rs gch l - sum q (l+v) + n (U + [n.v>0](1-k) Sb)
   const float nvpos = (st.nv > 0.f) ? 1.f : 0.f;
-  const float common = SPEC ? fmaf(nvpos * st.omk, acc[14], acc[12]) : 0.f;
-  const float Qs = SPEC ? acc[13] : 0.f;
-  pr.gn[3 * p + 0] = fmaf(st.vx, common, acc[6]);
-  pr.gn[3 * p + 1] = fmaf(st.vy, common, acc[7]);
-  pr.gn[3 * p + 2] = fmaf(st.vz, common, acc[8]);
+  const float g1v = SPEC ? nvpos * st.omk * acc[19] : 0.f;
+  pr.gn[3 * p + 0] = fmaf(st.vx, g1v, acc[6] + acc[12]);
+  pr.gn[3 * p + 1] = fmaf(st.vy, g1v, acc[7] + acc[13]);
+  pr.gn[3 * p + 2] = fmaf(st.vz, g1v, acc[8] + acc[14]);
   if (V != kDiffuse) {
     // disabled branches are written as zeros (bs:2889-2905, bs:3205-3214)
-    const float gvx = fmaf(-st.vx, Qs, fmaf(st.nx, common, acc[9]));
-    const float gvy = fmaf(-st.vy, Qs, fmaf(st.ny, common, acc[10]));
-    const float gvz = fmaf(-st.vz, Qs, fmaf(st.nz, common, acc[11]));
+    const float common = acc[18] + g1v;
+    const float gvx = fmaf(st.nx, common, acc[9] - acc[15]);
+    const float gvy = fmaf(st.ny, common, acc[10] - acc[16]);
+    const float gvz = fmaf(st.nz, common, acc[11] - acc[17]);
     pr.gv[3 * p + 0] = SPEC ? gvx : 0.f;
     pr.gv[3 * p + 1] = SPEC ? gvy : 0.f;
     pr.gv[3 * p + 2] = SPEC ? gvz : 0.f;
@@ -472,9 +484,9 @@ __device__ __forceinline__ void backward_tile(const Problem& pr, const Tiling& t
     float gr = 0.f;
     if (SPEC) {
       const float r = pr.rough[p];
-      const float gkt = fmaf(1.f - fmaxf(st.nv, 0.f), acc[14], acc[16]);
+      const float gkt = fmaf(1.f - fmaxf(st.nv, 0.f), acc[19], acc[21]);
       // bs:2501-2502
-      gr = fmaf(acc[15] * 4.f * r * r, r, gkt * (1.f + r) * 0.25f);
+      gr = fmaf(acc[20] * 4.f * r * r, r, gkt * (1.f + r) * 0.25f);
     }
     pr.gr[p] = gr;
   }

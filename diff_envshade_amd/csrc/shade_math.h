@@ -61,6 +61,7 @@ __device__ __forceinline__ TexelGeo texel_geometry(int t, int H, int W, const do
 struct Lobe {
   float cs;    // l.n
   float nl;    // max(cs, 0)
+  float hx, hy, hz;  // l + v, formed componentwise (exact for near-opposite l, v)
   float rs;    // 1/|l+v|
   float ch;    // l.h
   float nh;    // n.h
@@ -73,20 +74,23 @@ struct Lobe {
   bool spec_on;
 };
 
-// S provides nx..nz, vx..vz, vv, nv, k, omk, ib, a2, am1 (per-point constants).
+// S provides nx..nz, vx..vz, k, omk, ib, a2, am1 (per-point constants).
 template <bool SPEC, class S>
 __device__ __forceinline__ Lobe eval_lobe(const S& s, const float4& g) {
   Lobe o;
   o.cs = fmaf(g.z, s.nz, fmaf(g.y, s.ny, g.x * s.nx));
   o.nl = fmaxf(o.cs, 0.f);
   if (SPEC) {
-    const float lv = fmaf(g.z, s.vz, fmaf(g.y, s.vy, g.x * s.vx));
-    // |l+v|^2 = |l|^2 + 2 l.v + |v|^2;  h = (l+v)/|l+v|
-    const float hb2 = fmaf(2.f, lv, g.w + s.vv);
+    // h = (l+v)/|l+v| (bs:2218-2219); l+v componentwise keeps |l+v|, l.h and
+    // n.h accurate to fp32 rounding when l and v are nearly opposite
+    o.hx = g.x + s.vx;
+    o.hy = g.y + s.vy;
+    o.hz = g.z + s.vz;
+    const float hb2 = fmaf(o.hz, o.hz, fmaf(o.hy, o.hy, o.hx * o.hx));
     // floor keeps (n.h)^2 finite when v = -l exactly; such pairs are masked below
     o.rs = rsq(fmaxf(hb2, 1e-20f));
-    o.ch = (g.w + lv) * o.rs;
-    o.nh = (o.cs + s.nv) * o.rs;
+    o.ch = fmaf(g.z, o.hz, fmaf(g.y, o.hy, g.x * o.hx)) * o.rs;
+    o.nh = fmaf(s.nz, o.hz, fmaf(s.ny, o.hy, s.nx * o.hx)) * o.rs;
     const float om = 1.f - sat(o.ch);
     const float om2 = om * om;
     o.om4 = om2 * om2;
@@ -101,6 +105,8 @@ __device__ __forceinline__ Lobe eval_lobe(const S& s, const float4& g) {
     o.spec_on = (tmp > 0.f) && (hb2 > 0.f);
     o.tmp = o.spec_on ? tmp : 0.f;
   } else {
+    o.hx = o.hy = o.hz = 0.f;
+    o.rs = o.ch = o.nh = o.om4 = o.F = o.G = o.D = o.ia = o.it2 = o.tD = 0.f;
     o.spec_on = false;
     o.tmp = 0.f;
   }

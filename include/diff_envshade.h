@@ -252,6 +252,38 @@ DES_API int des_batch_forward(int32_t variant, int enable_diffuse, int enable_sp
 DES_API int des_batch_backward(int32_t variant, int enable_diffuse, int enable_specular,
                                int32_t ntc_backward, void* workspace, void* stream);
 
+/* ---- many env maps lighting one point set, on MFMA (no counterpart in the reference) ----
+ * shading[p, e, c] and diffuse_shading[p, e, c] (P x E x 3) for E env maps
+ * (E x 3 x H x W) sharing one rotation, with per-point visibility words
+ * (N_slot x P, bit t%32 of word [t/32][p]; NULL = all visible).  The per-pair
+ * Lambert and GGX weights are generated in registers in the MFMA operand
+ * layout and multiplied against the maps on the matrix cores.
+ * des_relight_backward_envmap gives grad_envmaps (E x 3 x H x W) from the
+ * upstream gradients (P x E x 3 each); the cross-block sum is FP64 in fixed
+ * order.  precision: DES_PREC_BF16, DES_PREC_FP16, or DES_PREC_BF16X3 (hi+lo
+ * split of both operands, three MFMAs per product). */
+enum { DES_PREC_BF16 = 0, DES_PREC_FP16 = 1, DES_PREC_BF16X3 = 2 };
+
+DES_API size_t des_relight_workspace_size(int64_t P, int32_t E, int32_t H, int32_t W,
+                                          int32_t precision, int with_envgrad);
+
+DES_API int des_relight_forward(int64_t P, int32_t E, int32_t H, int32_t W, int32_t N_slot,
+                                const float* normal, const float* view_dir, const float* albedo,
+                                const float* specular_albedo, const float* roughness,
+                                const float* envmaps, const float* light2obj_rotmat,
+                                const int32_t* visibility, int enable_specular, int32_t precision,
+                                float* shading, float* diffuse_shading, void* workspace,
+                                size_t workspace_bytes, void* stream);
+
+DES_API int des_relight_backward_envmap(int64_t P, int32_t E, int32_t H, int32_t W, int32_t N_slot,
+                                        const float* normal, const float* view_dir,
+                                        const float* albedo, const float* specular_albedo,
+                                        const float* roughness, const float* light2obj_rotmat,
+                                        const int32_t* visibility, const float* grad_shading,
+                                        const float* grad_diffuse_shading, int enable_specular,
+                                        int32_t precision, float* grad_envmaps, void* workspace,
+                                        size_t workspace_bytes, void* stream);
+
 /* Timing hook for benchmarks (no counterpart in the reference): when set,
  * the calling thread's next *_backward* call records hipEvent_t `start` right
  * before and `stop` right after its main shading kernel, on the call's stream,

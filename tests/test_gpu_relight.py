@@ -1,0 +1,110 @@
+"""Many-env-map relighting on MFMA (GPU): against the FP64 oracle per map and
+against the FP32 drop-in shader, for every operand precision."""
+import numpy as np
+import pytest
+import torch
+
+from _helpers import gpu_inputs, oracle_run, relerr
+from diff_envshade_amd import workloads as WL
+
+pytestmark = pytest.mark.gpu
+
+# normwise relative error vs FP64 allowed per precision (forward outputs and grad_envmap)
+TOLS = {"bf16x3": 2e-5, "bf16": 1e-2, "fp16": 2e-3}
+
+
+def relight_problem(seed, n, E, H, W, sigma=1.5):
+    rng = np.random.default_rng(seed)
+    d = WL.make_points(rng, n)
+    d["light2obj_rotmat"] = WL.random_rotation(rng)
+    d["visibility"] = WL.make_visibility(rng, d["normal"], H, W, d["light2obj_rotmat"])
+    envs = np.stack([WL.make_envmap(rng, H, W, sigma) for _ in range(E)])
+    gs = rng.normal(size=(n, E, 3)).astype(np.float32)
+    gd = rng.normal(size=(n, E, 3)).astype(np.float32)
+    return d, envs, gs, gd
+
+
+def per_map_oracle(d, envs, gs, gd):
+    n = d["normal"].shape[0]
+    sh, ds, gE = [], [], []
+    for e in range(envs.shape[0]):
+        de = dict(d)
+        de.update(WL.identity_mesh(n))
+        de["envmap"] = envs[e]
+        de["grad_shading"] = gs[:, e]
+        de["grad_diffuse_shading"] = gd[:, e]
+        (s, dd), b = oracle_run("vis", de)
+        sh.append(s)
+        ds.append(dd)
+        gE.append(b[-1])
+    return np.stack(sh, 1), np.stack(ds, 1), np.stack(gE)
+
+
+@pytest.mark.parametrize("precision", ["bf16x3", "bf16", "fp16"])
+def test_relight_matches_oracle(precision):
+    from diff_envshade_amd.relight import relight_backward_envmap, relight_forward
+    d, envs, gs, gd = relight_problem(0, n=1500, E=8, H=16, W=32)
+    t = gpu_inputs(d)
+    E = torch.from_numpy(envs).cuda()
+    sh, ds = relight_forward(t["normal"], t["view_dir"], t["albedo"], t["specular_albedo"],
+                             t["roughness"], E, t["light2obj_rotmat"], t["visibility"], True,
+                             precision)
+    gE = relight_backward_envmap(t["normal"], t["view_dir"], t["albedo"], t["specular_albedo"],
+                                 t["roughness"], 16, 32, t["light2obj_rotmat"], t["visibility"],
+                                 torch.from_numpy(gs).cuda(), torch.from_numpy(gd).cuda(), True,
+                                 precision)
+    torch.cuda.synchronize()
+    rsh, rds, rgE = per_map_oracle(d, envs, gs, gd)
+    errs = {"shading": relerr(sh.cpu().numpy(), rsh), "diffuse_shading": relerr(ds.cpu().numpy(), rds),
+            "grad_envmap": relerr(gE.cpu().numpy(), rgE)}
+    print(precision, errs)
+    assert max(errs.values()) <= TOLS[precision], errs
+
+
+def test_relight_many_maps_and_odd_sizes():
+    # 70 maps -> two column groups; 1001 points and a 3 x 5 map exercise padding
+    from diff_envshade_amd.relight import relight_backward_envmap, relight_forward
+    d, envs, gs, gd = relight_problem(1, n=1001, E=70, H=3, W=5)
+    t = gpu_inputs(d)
+    E = torch.from_numpy(envs).cuda()
+    sh, ds = relight_forward(t["normal"], t["view_dir"], t["albedo"], t["specular_albedo"],
+                             t["roughness"], E, t["light2obj_rotmat"], t["visibility"], True,
+                             "bf16x3")
+    gE = relight_backward_envmap(t["normal"], t["view_dir"], t["albedo"], t["specular_albedo"],
+                                 t["roughness"], 3, 5, t["light2obj_rotmat"], t["visibility"],
+                                 torch.from_numpy(gs).cuda(), torch.from_numpy(gd).cuda(), True,
+                                 "bf16x3")
+    torch.cuda.synchronize()
+    rsh, rds, rgE = per_map_oracle(d, envs, gs, gd)
+    assert relerr(sh.cpu().numpy(), rsh) <= TOLS["bf16x3"]
+    assert relerr(ds.cpu().numpy(), rds) <= TOLS["bf16x3"]
+    assert relerr(gE.cpu().numpy(), rgE) <= TOLS["bf16x3"]
+
+
+def test_relight_without_visibility_or_specular():
+    from diff_envshade_amd.relight import relight_forward
+    d, envs, gs, gd = relight_problem(2, n=700, E=4, H=8, W=16)
+    t = gpu_inputs(d)
+    E = torch.from_numpy(envs).cuda()
+    sh, ds = relight_forward(t["normal"], t["view_dir"], t["albedo"], t["specular_albedo"],
+                             t["roughness"], E, t["light2obj_rotmat"], None, False, "bf16x3")
+    torch.cuda.synchronize()
+    for e in range(4):
+        de = dict(d)
+        de["envmap"] = envs[e]
+        from oracle import oracle as O
+        r_sh, r_ds = O.forward("v2", de, True, False)
+        assert relerr(sh[:, e].cpu().numpy(), r_sh) <= TOLS["bf16x3"]
+        assert relerr(ds[:, e].cpu().numpy(), r_ds) <= TOLS["bf16x3"]
+
+
+def test_relight_envgrad_deterministic():
+    from diff_envshade_amd.relight import relight_backward_envmap
+    d, envs, gs, gd = relight_problem(3, n=3000, E=16, H=16, W=32)
+    t = gpu_inputs(d)
+    args = (t["normal"], t["view_dir"], t["albedo"], t["specular_albedo"], t["roughness"], 16, 32,
+            t["light2obj_rotmat"], t["visibility"], torch.from_numpy(gs).cuda(),
+            torch.from_numpy(gd).cuda(), True, "bf16")
+    a = relight_backward_envmap(*args).cpu().numpy()
+    b = relight_backward_envmap(*args).cpu().numpy()
+    assert np.array_equal(a, b)
