@@ -59,11 +59,23 @@ def main() -> int:
     ap.add_argument("--points", type=int, default=70000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--workload", default="dropin",
+                    choices=["dropin", "batched", "relight", "serving", "copy"],
+                    help="dropin (default, the headline); batched: 48 uneven point sets in one "
+                         "persistent launch; relight: 64 env maps x 262,144 points on MFMA; "
+                         "serving: Poisson arrivals through the continuous-batching server; "
+                         "copy: a plain 256 MiB device copy, the calibration kernel for the "
+                         "HBM byte counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the shader has no "
               "multi-GPU path in this build", file=sys.stderr)
         return 2
+    if args.workload != "dropin":
+        line = {"batched": bench_batched, "relight": bench_relight,
+                "serving": bench_serving, "copy": bench_copy}[args.workload](args)
+        print(json.dumps(line), flush=True)
+        return 0
 
     import numpy as np
     import torch
@@ -221,6 +233,226 @@ def main() -> int:
             }
     print(json.dumps(line), flush=True)
     return 0
+
+
+# ---- other workloads (not the headline) ---------------------------------------------
+
+def _oracle():
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from oracle import oracle as O
+    return O
+
+
+def _time_ms(fn, reps=10, warmup=2):
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+           for _ in range(reps)]
+    for a, b in evs:
+        a.record()
+        fn()
+        b.record()
+    torch.cuda.synchronize()
+    ts = sorted(a.elapsed_time(b) for a, b in evs)
+    return ts[len(ts) // 2], ts
+
+
+def bench_copy(args):
+    """Calibration for the FETCH_SIZE / WRITE_SIZE counters: a plain device copy
+    of 256 MiB (reads and writes exactly 256 MiB per launch)."""
+    import torch
+    n = 256 << 20
+    src = torch.empty(n // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+    dst = torch.empty_like(src)
+    reps = max(1, min(args.steps, 20))
+    ms, _ = _time_ms(lambda: dst.copy_(src), reps=reps)
+    return {"metric": "device copy bandwidth (counter calibration)", "value": 2 * n / (ms * 1e-3),
+            "unit": "B/s", "bytes_read_per_launch": n, "bytes_written_per_launch": n,
+            "launches": reps + 2, "ms": ms}
+
+
+def bench_batched(args):
+    """48 point sets, N log-uniform in [2k, 300k], 16x32 or 32x64 maps, fwd+bwd
+    fp32 in one persistent launch per direction, launch shape from the autotuner."""
+    import torch
+
+    from diff_envshade_amd import workloads as WL
+    from diff_envshade_amd.autotune import tune_batch
+    from diff_envshade_amd.batch import ShadeBatch
+    sets = WL.uneven_batch(seed=0, n_sets=48)
+    P = sum(int(s["normal"].shape[0]) for s in sets)
+    pairs = sum(int(s["normal"].shape[0]) * s["envmap"].shape[1] * s["envmap"].shape[2]
+                for s in sets)
+    print(f"[bench] batched: {len(sets)} sets, {P} points, {pairs / 1e9:.2f} G pairs",
+          file=sys.stderr, flush=True)
+    b = ShadeBatch("vis")
+    ntc_f, ntc_b = tune_batch(b, sets, reps=3)
+    b.capture()
+
+    ms, ts = _time_ms(b.replay, reps=max(5, args.steps // 20))
+    ms_eager, _ = _time_ms(lambda: (b.forward(), b.backward()), reps=5)
+    return {
+        "metric": "point·envmap pairs/s fwd+bwd, uneven batch of 48 sets in one persistent launch",
+        "value": P / (ms * 1e-3), "unit": "point·envmap pairs/s", "n_gpus": 1,
+        "ms_per_step": ms, "higher_is_better": True, "dtype": "fp32", "data": "synthetic",
+        "config": {"workload": "48 point sets, N log-uniform [2k, 300k], 16x32 or 32x64 env "
+                               "map each, 512/2048-bit visibility, forward+backward",
+                   "points": P, "point_texel_pairs": pairs, "ntc_forward": ntc_f,
+                   "ntc_backward": ntc_b, "graph": True},
+        "eager_ms_per_step": ms_eager,
+        "achieved_TFLOPs": pairs * (FLOP_FWD_PER_PAIR + FLOP_BWD_PER_PAIR) / (ms * 1e-3) / 1e12,
+    }
+
+
+def bench_relight(args):
+    """262,144 points x 64 env maps of 32x64 on MFMA: forward + env-map gradients,
+    per precision, against the fp32 drop-in on the same points and one map."""
+    import numpy as np
+    import torch
+
+    from diff_envshade_amd import _diff_envshade as C
+    from diff_envshade_amd import workloads as WL
+    from diff_envshade_amd.relight import relight_backward_envmap, relight_forward, workspace_size
+    dev = torch.device("cuda", 0)
+    P, E, H, W = 262144, 64, 32, 64
+    T = H * W
+    rng = np.random.default_rng(0)
+    d = WL.make_points(rng, P)
+    t = {k: torch.from_numpy(v).to(dev) for k, v in d.items()}
+    rot = torch.eye(3, device=dev)
+    vis = WL.make_visibility_device(t["normal"], H, W, seed=1)
+    envs = torch.from_numpy(np.stack([WL.make_envmap(rng, H, W, 1.5) for _ in range(E)])).to(dev)
+    gs = torch.randn((P, E, 3), device=dev, generator=torch.Generator(dev).manual_seed(2))
+    gd = torch.randn((P, E, 3), device=dev, generator=torch.Generator(dev).manual_seed(3))
+    mat = (t["normal"], t["view_dir"], t["albedo"], t["specular_albedo"], t["roughness"])
+    res = {}
+    O = None if args.no_accuracy else _oracle()
+    for prec in ("bf16", "fp16", "bf16x3"):
+        ws_f = torch.empty(workspace_size(P, E, H, W, prec), dtype=torch.uint8, device=dev)
+        ws_b = torch.empty(workspace_size(P, E, H, W, prec, True), dtype=torch.uint8, device=dev)
+        fwd = lambda: relight_forward(*mat, envs, rot, vis, True, prec, ws_f)  # noqa: E731
+        bwd = lambda: relight_backward_envmap(*mat, H, W, rot, vis, gs, gd, True, prec,  # noqa: E731
+                                              ws_b)
+        ms_f, _ = _time_ms(fwd, reps=5)
+        ms_b, _ = _time_ms(bwd, reps=5)
+        r = {"forward_ms": ms_f, "envgrad_ms": ms_b,
+             "pairs_per_s": P * E / ((ms_f + ms_b) * 1e-3)}
+        print(f"[bench] relight {prec}: fwd {ms_f:.3f} ms, envgrad {ms_b:.3f} ms",
+              file=sys.stderr, flush=True)
+        if O is not None:
+            # accuracy on a 256-point subset (forward) and a 2048-point problem (env grads)
+            n = 256
+            sub = {k: v[:n].contiguous() for k, v in t.items()}
+            sv = vis[:, :n].contiguous()
+            sh, ds = relight_forward(sub["normal"], sub["view_dir"], sub["albedo"],
+                                     sub["specular_albedo"], sub["roughness"], envs, rot, sv, True,
+                                     prec)
+            n2 = 2048
+            sub2 = {k: v[:n2].contiguous() for k, v in t.items()}
+            sv2 = vis[:, :n2].contiguous()
+            gE = relight_backward_envmap(sub2["normal"], sub2["view_dir"], sub2["albedo"],
+                                         sub2["specular_albedo"], sub2["roughness"], H, W, rot,
+                                         sv2, gs[:n2].contiguous(), gd[:n2].contiguous(), True,
+                                         prec)
+            errs = {"shading": 0.0, "diffuse_shading": 0.0, "grad_envmap": 0.0}
+            shn, dsn, gEn = sh.cpu().numpy(), ds.cpu().numpy(), gE.cpu().numpy()
+            base = {k: v.cpu().numpy() for k, v in sub.items()}
+            base2 = {k: v.cpu().numpy() for k, v in sub2.items()}
+            m1, m2 = WL.identity_mesh(n), WL.identity_mesh(n2)
+            num = {k: 0.0 for k in errs}
+            den = {k: 0.0 for k in errs}
+            for e in range(E):
+                de = dict(base, **m1, envmap=envs[e].cpu().numpy(), light2obj_rotmat=np.eye(3),
+                          visibility=sv.cpu().numpy())
+                rs, rd = O.forward("vis", de)
+                num["shading"] = max(num["shading"], np.abs(shn[:, e] - rs).max())
+                den["shading"] = max(den["shading"], np.abs(rs).max())
+                num["diffuse_shading"] = max(num["diffuse_shading"], np.abs(dsn[:, e] - rd).max())
+                den["diffuse_shading"] = max(den["diffuse_shading"], np.abs(rd).max())
+                de2 = dict(base2, **m2, envmap=envs[e].cpu().numpy(), light2obj_rotmat=np.eye(3),
+                           visibility=sv2.cpu().numpy())
+                g = O.backward("vis", de2, gs[:n2, e].cpu().numpy(), gd[:n2, e].cpu().numpy())
+                num["grad_envmap"] = max(num["grad_envmap"], np.abs(gEn[e] - g["envmap"]).max())
+                den["grad_envmap"] = max(den["grad_envmap"], np.abs(g["envmap"]).max())
+            r["err_vs_fp64_max_rel"] = {k: num[k] / den[k] for k in num}
+        res[prec] = r
+        del ws_f, ws_b
+    # the fp32 drop-in on the same points and the first map, forward + backward
+    mesh = {k: torch.from_numpy(v).to(dev) for k, v in WL.identity_mesh(P).items()}
+    g1 = gs[:, 0].contiguous()
+    g2 = gd[:, 0].contiguous()
+    e0 = envs[0].contiguous()
+
+    def dropin():
+        C.bruteforce_specularvisibility_shader_forward2(*mat, e0, rot, mesh["face_vertex_list"],
+                                                        mesh["nearest_triangle_id"],
+                                                        mesh["barycentric_coord"], vis, True)
+        C.bruteforce_specularvisibility_shader_backward2(g1, g2, *mat, e0, rot,
+                                                         mesh["face_vertex_list"],
+                                                         mesh["nearest_triangle_id"],
+                                                         mesh["barycentric_coord"], vis, True)
+    ms_d, _ = _time_ms(dropin, reps=5)
+    dropin_rate = P / (ms_d * 1e-3)
+    best = res["bf16"]["pairs_per_s"]
+    return {
+        "metric": "point·envmap pairs/s, forward + env-map gradients, 64 maps on MFMA",
+        "value": best, "unit": "point·envmap pairs/s", "n_gpus": 1,
+        "ms_per_step": res["bf16"]["forward_ms"] + res["bf16"]["envgrad_ms"],
+        "higher_is_better": True, "dtype": "bf16 operands, fp32 accumulate", "data": "synthetic",
+        "config": {"workload": "262,144 points with 2048-bit visibility x 64 env maps 32x64 "
+                               "lognormal(0,1.5): forward + env-map gradients",
+                   "points": P, "maps": E, "env_h": H, "env_w": W},
+        "precisions": res,
+        "fp32_dropin": {"ms_fwd_bwd_one_map": ms_d, "pairs_per_s": dropin_rate},
+        "speedup_vs_fp32_dropin": {k: v["pairs_per_s"] / dropin_rate for k, v in res.items()},
+    }
+
+
+def bench_serving(args):
+    """2000 Poisson requests, N log-uniform [2k, 300k], 30% forward-only, env maps
+    from a 32-map device pool (16x32 / 32x64), 64 GB HBM admission budget."""
+    import numpy as np
+    import torch
+
+    from diff_envshade_amd import workloads as WL
+    from diff_envshade_amd.batch import ShadeBatch
+    from diff_envshade_amd.serving import (DevicePool, GpuExecutor, PoissonWorkload, Scheduler,
+                                           summarize)
+    shapes = [(16, 32), (32, 64)] * 16
+    pool = DevicePool(max_points=300000, env_shapes=shapes, seed=0)
+    # capacity: one batch of 48 requests drawn from the same distribution
+    probe = PoissonWorkload(n_requests=48, rate=1e9, n_env=32, seed=11)
+    preqs = probe.requests(lambda r, e: pool.request_payload(r, e, 0))
+    b = ShadeBatch("vis")
+    fw = [r.payload for r in preqs]
+    b.prepare(fw, with_backward=True)
+    ms, _ = _time_ms(lambda: (b.forward(), b.backward()), reps=3)
+    cap_req_s = len(preqs) / (ms * 1e-3)
+    rate = 0.75 * cap_req_s
+    print(f"[bench] serving: capacity ~{cap_req_s:.0f} req/s, arrival rate {rate:.0f} req/s",
+          file=sys.stderr, flush=True)
+    wl = PoissonWorkload(n_requests=2000, rate=rate, n_env=32, seed=12)
+    rng = np.random.default_rng(13)
+    offs = rng.integers(0, 300000, size=2000)
+    reqs = wl.requests(lambda r, e: (lambda: pool.request_payload(r, e, int(offs[r.rid]))))
+    ex = GpuExecutor("vis", slots=3)
+    sch = Scheduler(ex, hbm_budget=64 << 30)
+    sch.run(reqs)
+    torch.cuda.synchronize()
+    s = summarize(reqs, lambda r: r.points)
+    return {
+        "metric": "serving: p50/p99 latency and aggregate point·envmap pairs/s",
+        "value": s["pairs_per_s"], "unit": "point·envmap pairs/s", "n_gpus": 1,
+        "higher_is_better": True, "dtype": "fp32", "data": "synthetic",
+        "config": {"workload": "2000 Poisson requests, N log-uniform [2k, 300k], 30% forward-"
+                               "only, 32 device-resident 16x32/32x64 maps, 64 GB budget",
+                   "arrival_rate_req_s": rate, "capacity_req_s": cap_req_s, "slots": 3},
+        "latency_ms": {"p50": s["p50_ms"], "p99": s["p99_ms"], "max": s["max_ms"]},
+        "makespan_s": s["makespan_s"], "batches": sch.batches,
+        "mean_batch": float(np.mean(sch.batch_sizes)), "peak_in_use_bytes": sch.peak_in_use,
+    }
 
 
 if __name__ == "__main__":

@@ -1,7 +1,7 @@
 """Launch-shape autotuner for the shading kernels.
 
 The only free launch parameter is NTC, the number of waves that split the
-texel range of one 64-point tile (1..16 forward, 1..8 backward).  Small
+texel range of one 64-point tile (1..8).  Small
 point sets want many waves per tile to fill 256 CUs; large ones want few, to
 save the per-wave setup and the cross-wave reduction.  The tuner times every
 candidate on the device with HIP events and keeps the fastest, per batch
@@ -19,7 +19,7 @@ import threading
 
 import torch
 
-FWD_CANDIDATES = (1, 2, 4, 8, 16)
+FWD_CANDIDATES = (1, 2, 4, 8)
 BWD_CANDIDATES = (1, 2, 4, 8)
 
 _cache: dict = {}

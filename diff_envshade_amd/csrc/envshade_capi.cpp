@@ -63,7 +63,7 @@ des::Tiling tiling_for(int ntc, int T, bool backward) {
   tl.ntc = ntc;
   tl.cw = slots_per_wave * 32;
   tl.st = ntc * tl.cw;
-  const size_t table = size_t(32) * tl.st;
+  const size_t table = size_t(des::kTexelBytes) * tl.st;
   tl.lds_fwd = std::max(table, size_t(ntc) * des::kForwardAcc * 64 * sizeof(float));
   tl.lds_bwd = std::max(table, size_t(ntc) * des::kBackwardAcc * 64 * sizeof(float));
   tl.reduce_rows = des::kReduceRows;
@@ -545,7 +545,7 @@ DES_API int des_batch_prepare(int32_t variant, int enable_diffuse, int enable_sp
     return fail(DES_ERR_INVALID_ARGUMENT, "unknown variant %d", variant);
   if (n_sets < 0 || (n_sets > 0 && sets == nullptr))
     return fail(DES_ERR_INVALID_ARGUMENT, "sets must hold n_sets >= 0 descriptors");
-  if (int rc = check_ntc(ntc_forward, 16, "ntc_forward")) return rc;
+  if (int rc = check_ntc(ntc_forward, 8, "ntc_forward")) return rc;
   if (int rc = check_ntc(ntc_backward, 8, "ntc_backward")) return rc;
   const bool diff = variant == des::kDiffuse || variant == des::kV2 || variant == des::kVis || enable_diffuse;
   const bool spec = variant != des::kDiffuse && enable_specular;
@@ -611,7 +611,7 @@ DES_API int des_batch_forward(int32_t variant, int enable_diffuse, int enable_sp
                               int32_t ntc_forward, void* workspace, void* stream) {
   if (variant < des::kDiffuse || variant > des::kVis)
     return fail(DES_ERR_INVALID_ARGUMENT, "unknown variant %d", variant);
-  if (int rc = check_ntc(ntc_forward, 16, "ntc_forward")) return rc;
+  if (int rc = check_ntc(ntc_forward, 8, "ntc_forward")) return rc;
   REQUIRE(workspace);
   const bool diff = variant == des::kDiffuse || variant == des::kV2 || variant == des::kVis || enable_diffuse;
   const bool spec = variant != des::kDiffuse && enable_specular;

@@ -71,38 +71,30 @@ __device__ __forceinline__ float reduce4(float q0, float q1, float q2, float q3)
 //   dOmega = (sin(pi(iy+1)/H - pi/2) - sin(pi iy/H - pi/2)) 2pi/W
 //          = 2 sin(pi/(2H)) cos(theta) 2pi/W   (exact identity, no cancellation)
 //   l = M d with M row-major.
-__device__ void build_table(const Problem& pr, int t0, int n, int n_alloc, float4* geo,
-                            float4* rad) {
+// Texels [t0, t0+n) of the lat-long map into LDS entries [0, n); entries
+// [n, n_alloc) are zero so padded texels contribute exactly nothing.
+__device__ void build_table(const Problem& pr, int t0, int n, int n_alloc, Texel* tab) {
   double M[9];
 #pragma unroll
   for (int i = 0; i < 9; ++i) M[i] = (double)pr.rotmat[i];
   const double kdw = solid_angle_scale(pr.H, pr.W);
   const int T = pr.T;
   for (int i = threadIdx.x; i < n_alloc; i += blockDim.x) {
-    float4 g = make_float4(0.f, 0.f, 0.f, 0.f);
-    float4 e = make_float4(0.f, 0.f, 0.f, 0.f);
     if (i < n) {
       const int t = t0 + i;
-      const TexelGeo tg = texel_geometry(t, pr.H, pr.W, M, kdw);
-      g = make_float4(tg.lx, tg.ly, tg.lz, tg.ll);
-      e = make_float4((float)(pr.envmap[t] * tg.dw), (float)(pr.envmap[T + t] * tg.dw),
-                      (float)(pr.envmap[2 * T + t] * tg.dw), (float)tg.dw);
+      tab[i] = make_texel(texel_geometry(t, pr.H, pr.W, M, kdw), pr.envmap[t], pr.envmap[T + t],
+                          pr.envmap[2 * T + t]);
+    } else {
+      tab[i] = zero_texel();
     }
-    geo[i] = g;
-    rad[i] = e;
   }
 }
 
 // ---- per-point state ---------------------------------------------------------
 
 template <int V, bool SPEC>
-struct PointState {
-  float nx, ny, nz;
-  float vx, vy, vz;
-  float vv, nv;        // |v|^2, n.v (unclamped)
-  float k, omk, ib;    // Smith-Schlick k, 1-k, 1/(max(n.v,0)(1-k)+k)
-  float a2, am1;       // alpha^2 = r^4, alpha^2 - 1
-  float live;          // 0 for padding lanes and culled points (clamp variant)
+struct PointState : PointGeom {
+  float live;  // 0 for padding lanes and culled points (clamp variant)
   // visibility
   int vert0, vert1, vert2;
   float c0, c1, c2;
@@ -111,29 +103,19 @@ struct PointState {
 template <int V, bool SPEC>
 __device__ __forceinline__ void load_point(const Problem& pr, int64_t p, bool valid,
                                            PointState<V, SPEC>& s) {
-  s.nx = s.ny = s.nz = 0.f;
-  s.vx = s.vy = s.vz = 0.f;
-  float r = 0.f;
+  float nx = 0.f, ny = 0.f, nz = 0.f, vx = 0.f, vy = 0.f, vz = 0.f, r = 0.f;
   if (valid) {
-    s.nx = pr.normal[3 * p + 0];
-    s.ny = pr.normal[3 * p + 1];
-    s.nz = pr.normal[3 * p + 2];
+    nx = pr.normal[3 * p + 0];
+    ny = pr.normal[3 * p + 1];
+    nz = pr.normal[3 * p + 2];
     if (SPEC || V == kClamp) {
-      s.vx = pr.view_dir[3 * p + 0];
-      s.vy = pr.view_dir[3 * p + 1];
-      s.vz = pr.view_dir[3 * p + 2];
+      vx = pr.view_dir[3 * p + 0];
+      vy = pr.view_dir[3 * p + 1];
+      vz = pr.view_dir[3 * p + 2];
     }
     if (SPEC) r = pr.rough[p];
   }
-  s.vv = s.vx * s.vx + s.vy * s.vy + s.vz * s.vz;
-  s.nv = s.nx * s.vx + s.ny * s.vy + s.nz * s.vz;
-  // bs:2126-2129: alpha2 = r^4, k = (r+1)^2 / 8
-  const float r2 = r * r;
-  s.a2 = r2 * r2;
-  s.am1 = s.a2 - 1.f;
-  s.k = (r + 1.f) * (r + 1.f) * 0.125f;
-  s.omk = 1.f - s.k;
-  s.ib = 1.f / (fmaxf(s.nv, 0.f) * s.omk + s.k);
+  static_cast<PointGeom&>(s) = point_geom(nx, ny, nz, vx, vy, vz, r);
   s.live = valid ? 1.f : 0.f;
   if (V == kClamp && !(s.nv > 0.f)) s.live = 0.f;  // per-point back-face cull (bs:1111)
   s.vert0 = s.vert1 = s.vert2 = 0;
@@ -220,12 +202,11 @@ __device__ __forceinline__ void forward_tile(const Problem& pr, const Tiling& tl
   float ds0 = 0.f, ds1 = 0.f, ds2 = 0.f;  // sum E dOmega vis max(l.n,0)
   float sp0 = 0.f, sp1 = 0.f, sp2 = 0.f;  // sum E dOmega vis F G D / 4
 
-  float4* geo = smem;
-  float4* rad = smem + tl.st;
+  Texel* tab = reinterpret_cast<Texel*>(smem);
   for (int s0 = 0; s0 < pr.T; s0 += tl.st) {
     const int sn = min(tl.st, pr.T - s0);
     __syncthreads();
-    build_table(pr, s0, sn, tl.st, geo, rad);
+    build_table(pr, s0, sn, tl.st, tab);
     __syncthreads();
     const int c_begin = wv * tl.cw;
     const int c_end = min(c_begin + tl.cw, (sn + 31) & ~31);
@@ -233,21 +214,20 @@ __device__ __forceinline__ void forward_tile(const Problem& pr, const Tiling& tl
       const VisWords w = load_slot<V, SPEC>(pr, st, (s0 + sl) >> 5, single);
 #pragma unroll 4
       for (int b = 0; b < 32; ++b) {
-        const float4 g = geo[sl + b];
-        const float4 e = rad[sl + b];
+        const Texel& g = tab[sl + b];
         const float vis = vis_weight<V, SPEC>(st, w, b, single, csum);
         const Lobe lb = eval_lobe<SPEC>(st, g);
         if (DIFF) {
           const float x = lb.nl * vis;
-          ds0 = fmaf(e.x, x, ds0);
-          ds1 = fmaf(e.y, x, ds1);
-          ds2 = fmaf(e.z, x, ds2);
+          ds0 = fmaf(g.e0, x, ds0);
+          ds1 = fmaf(g.e1, x, ds1);
+          ds2 = fmaf(g.e2, x, ds2);
         }
         if (SPEC) {
           const float y = lb.tmp * vis;
-          sp0 = fmaf(e.x, y, sp0);
-          sp1 = fmaf(e.y, y, sp1);
-          sp2 = fmaf(e.z, y, sp2);
+          sp0 = fmaf(g.e0, y, sp0);
+          sp1 = fmaf(g.e1, y, sp1);
+          sp2 = fmaf(g.e2, y, sp2);
         }
       }
     }
@@ -297,9 +277,9 @@ __device__ __forceinline__ void forward_tile(const Problem& pr, const Tiling& tl
 //   gE_c(t)         = dOmega_t * sum_p (A_c x + B_c y)
 //   l-coefficients of grad_normal / grad_view_dir come from the diffuse term,
 //   the G1 VJP (bs:173-201) and the normalize VJP (bs:62-71); the half-vector
-//   terms of the NDF and normalize VJPs are accumulated along l + v; the
-//   n- and v-directed parts are per-point constants times scalar sums (U, Sb),
-//   applied once at the end.
+//   terms of the NDF and normalize VJPs are accumulated along l + v and along
+//   hb x (n x hb); the G1 parts along n and v are per-point constants times a
+//   scalar sum (Sb), applied once at the end.
 template <int V, bool DIFF, bool SPEC>
 __device__ __forceinline__ void backward_tile(const Problem& pr, const Tiling& tl, int64_t tile) {
   extern __shared__ float4 smem[];
@@ -336,11 +316,11 @@ __device__ __forceinline__ void backward_tile(const Problem& pr, const Tiling& t
   float lnx = 0.f, lny = 0.f, lnz = 0.f;  // sum coef_n l
   float lvx = 0.f, lvy = 0.f, lvz = 0.f;  // sum coef_v l
   float hnx = 0.f, hny = 0.f, hnz = 0.f;  // sum rs gdnh (l + v)
-  float hvx = 0.f, hvy = 0.f, hvz = 0.f;  // sum q (l + v)
-  float U = 0.f, Sb = 0.f, ga2 = 0.f, gk = 0.f;
+  float hvx = 0.f, hvy = 0.f, hvz = 0.f;  // sum rs^2 gch (l.h) (l + v)
+  float cvx = 0.f, cvy = 0.f, cvz = 0.f;  // sum rs^3 gdnh (hb x (n x hb))
+  float Sb = 0.f, ga2 = 0.f, gk = 0.f;
 
-  float4* geo = smem;
-  float4* rad = smem + tl.st;
+  Texel* tab = reinterpret_cast<Texel*>(smem);
   float* part = pr.part + tile * 3 * pr.T;
   // lanes (lane & 15) < 3 of row k store channel (lane & 15) of texel kRowTexel[k]
   const int row = lane >> 4;
@@ -350,7 +330,7 @@ __device__ __forceinline__ void backward_tile(const Problem& pr, const Tiling& t
   for (int s0 = 0; s0 < pr.T; s0 += tl.st) {
     const int sn = min(tl.st, pr.T - s0);
     __syncthreads();
-    build_table(pr, s0, sn, tl.st, geo, rad);
+    build_table(pr, s0, sn, tl.st, tab);
     __syncthreads();
     const int c_begin = wv * tl.cw;
     const int c_end = min(c_begin + tl.cw, (sn + 31) & ~31);
@@ -362,27 +342,26 @@ __device__ __forceinline__ void backward_tile(const Problem& pr, const Tiling& t
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
           const int b = b4 + i;
-          const float4 g = geo[sl + b];
-          const float4 e = rad[sl + b];
+          const Texel& g = tab[sl + b];
           const float vis = vis_weight<V, SPEC>(st, w, b, single, csum);
           const Lobe lb = eval_lobe<SPEC>(st, g);
           float coef_n = 0.f;
           float x = 0.f, y = 0.f;
           if (DIFF) {
             x = lb.nl * vis;
-            ds0 = fmaf(e.x, x, ds0);
-            ds1 = fmaf(e.y, x, ds1);
-            ds2 = fmaf(e.z, x, ds2);
+            ds0 = fmaf(g.e0, x, ds0);
+            ds1 = fmaf(g.e1, x, ds1);
+            ds2 = fmaf(g.e2, x, ds2);
             // gdot = vis [l.n > 0] sum_c A_c E'_c   (bs:2426-2438)
-            const float AE = fmaf(A[2], e.z, fmaf(A[1], e.y, A[0] * e.x));
+            const float AE = fmaf(A[2], g.e2, fmaf(A[1], g.e1, A[0] * g.e0));
             coef_n = (lb.cs > 0.f) ? AE * vis : 0.f;
           }
           if (SPEC) {
             y = lb.tmp * vis;
-            sp0 = fmaf(e.x, y, sp0);
-            sp1 = fmaf(e.y, y, sp1);
-            sp2 = fmaf(e.z, y, sp2);
-            const float BE = fmaf(B[2], e.z, fmaf(B[1], e.y, B[0] * e.x));
+            sp0 = fmaf(g.e0, y, sp0);
+            sp1 = fmaf(g.e1, y, sp1);
+            sp2 = fmaf(g.e2, y, sp2);
+            const float BE = fmaf(B[2], g.e2, fmaf(B[1], g.e1, B[0] * g.e0));
             const float gt = lb.spec_on ? BE * vis : 0.f;  // grad of F G D / 4
             const float gq = 0.25f * gt;
             const float gF = gq * lb.G * lb.D;
@@ -407,12 +386,16 @@ __device__ __forceinline__ void backward_tile(const Problem& pr, const Tiling& t
             // h = hb rs with hb = l + v; grad_h = gch l + gdnh n.
             //   NDF term of grad_normal:   gdnh h = (rs gdnh) hb
             //   normalize VJP (bs:62-71):  rs (grad_h - h (h.grad_h))
-            //                            = rs gch l + rs gdnh n - q hb
-            // hb is accumulated as a vector: splitting it into l and v parts
-            // would cancel by ~1/|l+v| when l and v are nearly opposite.
+            //     = rs gch (l - (l.h) h) + rs gdnh (n - (n.h) h)
+            //     = rs gch l - rs^2 gch (l.h) hb + rs^3 gdnh (hb x (n x hb))
+            // The last form of n - (n.h) h (size sin(theta_h)) avoids the
+            // 1/sin(theta_h) cancellation of sharp lobes; hb is accumulated as a
+            // vector because splitting it into l and v parts would cancel by
+            // ~1/|l+v| when l and v are nearly opposite.
             const float rg = lb.rs * gdnh;
-            const float q2 = lb.rs * lb.rs * fmaf(gch, lb.ch, gdnh * lb.nh);
-            U += rg;
+            const float rs2 = lb.rs * lb.rs;
+            const float q2 = rs2 * gch * lb.ch;
+            const float r3 = rs2 * rg;
             coef_n += gdl;
             hnx = fmaf(rg, lb.hx, hnx);
             hny = fmaf(rg, lb.hy, hny);
@@ -420,14 +403,17 @@ __device__ __forceinline__ void backward_tile(const Problem& pr, const Tiling& t
             hvx = fmaf(q2, lb.hx, hvx);
             hvy = fmaf(q2, lb.hy, hvy);
             hvz = fmaf(q2, lb.hz, hvz);
+            cvx = fmaf(r3, fmaf(lb.hy, lb.cz, -lb.hz * lb.cy), cvx);
+            cvy = fmaf(r3, fmaf(lb.hz, lb.cx, -lb.hx * lb.cz), cvy);
+            cvz = fmaf(r3, fmaf(lb.hx, lb.cy, -lb.hy * lb.cx), cvz);
             const float coef_v = lb.rs * gch;
-            lvx = fmaf(coef_v, g.x, lvx);
-            lvy = fmaf(coef_v, g.y, lvy);
-            lvz = fmaf(coef_v, g.z, lvz);
+            lvx = fmaf(coef_v, g.lx, lvx);
+            lvy = fmaf(coef_v, g.ly, lvy);
+            lvz = fmaf(coef_v, g.lz, lvz);
           }
-          lnx = fmaf(coef_n, g.x, lnx);
-          lny = fmaf(coef_n, g.y, lny);
-          lnz = fmaf(coef_n, g.z, lnz);
+          lnx = fmaf(coef_n, g.lx, lnx);
+          lny = fmaf(coef_n, g.ly, lny);
+          lnz = fmaf(coef_n, g.lz, lnz);
 #pragma unroll
           for (int c = 0; c < 3; ++c) q[i][c] = fmaf(A[c], x, B[c] * y);
         }
@@ -446,8 +432,8 @@ __device__ __forceinline__ void backward_tile(const Problem& pr, const Tiling& t
 
   // fixed-order reduction of the per-point sums over the NTC texel chunks
   constexpr int NACC = kBackwardAcc;
-  float acc[NACC] = {ds0, ds1, ds2, sp0, sp1, sp2, lnx, lny, lnz, lvx, lvy,
-                     lvz, hnx, hny, hnz, hvx, hvy, hvz, U,   Sb,  ga2, gk};
+  float acc[NACC] = {ds0, ds1, ds2, sp0, sp1, sp2, lnx, lny, lnz, lvx, lvy, lvz,
+                     hnx, hny, hnz, hvx, hvy, hvz, cvx, cvy, cvz, Sb,  ga2, gk};
   if (tl.ntc > 1) {
     __syncthreads();
     float* red = reinterpret_cast<float*>(smem);
@@ -464,18 +450,18 @@ __device__ __forceinline__ void backward_tile(const Problem& pr, const Tiling& t
   }
   if (!valid) return;
   // gn = sum coef_n l + sum rs gdnh (l+v) + v [n.v>0](1-k) Sb
-  // gv = sum rs gch l - sum q (l+v) + n (U + [n.v>0](1-k) Sb)
+  // gv = sum rs gch l - sum rs^2 gch (l.h) (l+v) + sum rs^3 gdnh (hb x (n x hb))
+  //      + n [n.v>0](1-k) Sb
   const float nvpos = (st.nv > 0.f) ? 1.f : 0.f;
-  const float g1v = SPEC ? nvpos * st.omk * acc[19] : 0.f;
+  const float g1v = SPEC ? nvpos * st.omk * acc[21] : 0.f;
   pr.gn[3 * p + 0] = fmaf(st.vx, g1v, acc[6] + acc[12]);
   pr.gn[3 * p + 1] = fmaf(st.vy, g1v, acc[7] + acc[13]);
   pr.gn[3 * p + 2] = fmaf(st.vz, g1v, acc[8] + acc[14]);
   if (V != kDiffuse) {
     // disabled branches are written as zeros (bs:2889-2905, bs:3205-3214)
-    const float common = acc[18] + g1v;
-    const float gvx = fmaf(st.nx, common, acc[9] - acc[15]);
-    const float gvy = fmaf(st.ny, common, acc[10] - acc[16]);
-    const float gvz = fmaf(st.nz, common, acc[11] - acc[17]);
+    const float gvx = fmaf(st.nx, g1v, (acc[9] - acc[15]) + acc[18]);
+    const float gvy = fmaf(st.ny, g1v, (acc[10] - acc[16]) + acc[19]);
+    const float gvz = fmaf(st.nz, g1v, (acc[11] - acc[17]) + acc[20]);
     pr.gv[3 * p + 0] = SPEC ? gvx : 0.f;
     pr.gv[3 * p + 1] = SPEC ? gvy : 0.f;
     pr.gv[3 * p + 2] = SPEC ? gvz : 0.f;
@@ -484,9 +470,9 @@ __device__ __forceinline__ void backward_tile(const Problem& pr, const Tiling& t
     float gr = 0.f;
     if (SPEC) {
       const float r = pr.rough[p];
-      const float gkt = fmaf(1.f - fmaxf(st.nv, 0.f), acc[19], acc[21]);
+      const float gkt = fmaf(1.f - fmaxf(st.nv, 0.f), acc[21], acc[23]);
       // bs:2501-2502
-      gr = fmaf(acc[20] * 4.f * r * r, r, gkt * (1.f + r) * 0.25f);
+      gr = fmaf(acc[22] * 4.f * r * r, r, gkt * (1.f + r) * 0.25f);
     }
     pr.gr[p] = gr;
   }
@@ -533,7 +519,7 @@ __device__ __forceinline__ int64_t next_item(unsigned long long* counter) {
 }
 
 template <int V, bool DIFF, bool SPEC>
-__global__ __launch_bounds__(1024) void batch_forward_kernel(const BatchHeader* __restrict__ hdr,
+__global__ __launch_bounds__(512) void batch_forward_kernel(const BatchHeader* __restrict__ hdr,
                                                              const BatchSet* __restrict__ sets,
                                                              unsigned long long* counter) {
   const int64_t total = hdr->total_tiles;
@@ -543,9 +529,7 @@ __global__ __launch_bounds__(1024) void batch_forward_kernel(const BatchHeader* 
     if (g >= total) break;
     const int s = __builtin_amdgcn_readfirstlane(
         find_set(n, g, [&](int i) { return sets[i].tile0; }));
-    const Problem pr = sets[s].pr;
-    const Tiling tl = sets[s].tl_fwd;
-    forward_tile<V, DIFF, SPEC>(pr, tl, g - sets[s].tile0);
+    forward_tile<V, DIFF, SPEC>(sets[s].pr, sets[s].tl_fwd, g - sets[s].tile0);
   }
 }
 
@@ -561,9 +545,7 @@ __global__ __launch_bounds__(512) void batch_backward_kernel(const BatchHeader* 
     if (g >= total) break;
     const int s = __builtin_amdgcn_readfirstlane(
         find_set(n, g, [&](int i) { return sets[i].tile0; }));
-    const Problem pr = sets[s].pr;
-    const Tiling tl = sets[s].tl_bwd;
-    backward_tile<V, DIFF, SPEC>(pr, tl, g - sets[s].tile0);
+    backward_tile<V, DIFF, SPEC>(sets[s].pr, sets[s].tl_bwd, g - sets[s].tile0);
   }
 }
 

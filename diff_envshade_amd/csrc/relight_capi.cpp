@@ -45,13 +45,13 @@ RelightLayout relight_layout(int64_t P, int E, int H, int W, int prec, bool envg
   L.npc = std::max(1, (L.kp + L.ks_per_chunk - 1) / L.ks_per_chunk);
   size_t off = 0;
   L.geo = off;
-  off += a256(size_t(L.tp) * 16);
+  off += a256(size_t(L.tp) * sizeof(des::math::Texel));
   L.lfrag = off;
   off += a256(size_t(L.kc) * L.ntg_max * L.nf * 64 * 16);
   L.pconst = L.abfrag = L.part = off;
   if (envgrad) {
     L.pconst = off;
-    off += a256(size_t(L.kp) * 32 * 3 * 16);
+    off += a256(size_t(L.kp) * 32 * sizeof(des::math::PointGeom));
     L.abfrag = off;
     off += a256(size_t(L.kp) * L.ntg_max * 2 * L.nf * 64 * 16);
     L.part = off;
@@ -91,9 +91,9 @@ RelightArgs make_args(const RelightLayout& L, int64_t P, int E, int H, int W, vo
   a.npc = L.npc;
   a.ks_per_chunk = L.ks_per_chunk;
   char* w = static_cast<char*>(ws);
-  a.geo = reinterpret_cast<float4*>(w + L.geo);
+  a.geo = reinterpret_cast<des::math::Texel*>(w + L.geo);
   a.lfrag = reinterpret_cast<uint4*>(w + L.lfrag);
-  a.pconst = reinterpret_cast<float4*>(w + L.pconst);
+  a.pconst = reinterpret_cast<des::math::PointGeom*>(w + L.pconst);
   a.abfrag = reinterpret_cast<uint4*>(w + L.abfrag);
   a.part = reinterpret_cast<float*>(w + L.part);
   return a;

@@ -12,6 +12,8 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include "shade_math.h"
+
 namespace des {
 
 enum RelightPrecision : int {
@@ -50,9 +52,9 @@ struct RelightArgs {
   float* ds;              // P x E x 3
   float* gE;              // E x 3 x H x W
   // workspace
-  float4* geo;            // tp entries: (lx, ly, lz, |l|^2)
+  math::Texel* geo;       // tp texel entries
   uint4* lfrag;           // [kc][ntg][nf][64] B-operand fragments of L
-  float4* pconst;         // [kp*32][3] per-point constants
+  math::PointGeom* pconst;  // kp*32 per-point constants
   uint4* abfrag;          // [kp][ntg][2][nf][64] B-operand fragments of A and B
   float* part;            // [npc][tp][ntg*16]
 };

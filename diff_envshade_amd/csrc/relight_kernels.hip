@@ -63,32 +63,9 @@ __device__ __forceinline__ void to_frag(const float* w, uint4& hi, uint4& lo) {
   }
 }
 
-// Per-point constants of the lobe (fields read by math::eval_lobe).
-struct PointC {
-  float nx, ny, nz, nv;
-  float vx, vy, vz, vv;
-  float k, omk, ib, a2, am1;
-};
-
-__device__ __forceinline__ PointC point_consts(float nx, float ny, float nz, float vx, float vy,
-                                               float vz, float r) {
-  PointC c;
-  c.nx = nx; c.ny = ny; c.nz = nz;
-  c.vx = vx; c.vy = vy; c.vz = vz;
-  c.vv = vx * vx + vy * vy + vz * vz;
-  c.nv = nx * vx + ny * vy + nz * vz;
-  const float r2 = r * r;  // alpha^2 = r^4, k = (r+1)^2/8 (bruteforce_shader.cu:2126-2129)
-  c.a2 = r2 * r2;
-  c.am1 = c.a2 - 1.f;
-  c.k = (r + 1.f) * (r + 1.f) * 0.125f;
-  c.omk = 1.f - c.k;
-  c.ib = 1.f / (fmaxf(c.nv, 0.f) * c.omk + c.k);
-  return c;
-}
-
 // Padding points get benign constants (finite lobe) and zero visibility.
-__device__ __forceinline__ PointC load_point_consts(const RelightArgs& a, int64_t p, bool spec) {
-  if (p >= a.P) return point_consts(0.f, 0.f, 0.f, 0.f, 0.f, 1.f, 0.5f);
+__device__ __forceinline__ PointGeom load_point_consts(const RelightArgs& a, int64_t p, bool spec) {
+  if (p >= a.P) return point_geom(0.f, 0.f, 0.f, 0.f, 0.f, 1.f, 0.5f);
   const float nx = a.normal[3 * p], ny = a.normal[3 * p + 1], nz = a.normal[3 * p + 2];
   float vx = 0.f, vy = 0.f, vz = 1.f, r = 0.5f;
   if (spec) {
@@ -97,11 +74,11 @@ __device__ __forceinline__ PointC load_point_consts(const RelightArgs& a, int64_
     vz = a.view_dir[3 * p + 2];
     r = a.rough[p];
   }
-  return point_consts(nx, ny, nz, vx, vy, vz, r);
+  return point_geom(nx, ny, nz, vx, vy, vz, r);
 }
 
 template <bool SPEC>
-__device__ __forceinline__ void pair_weights(const PointC& pc, const float4& g, float vis,
+__device__ __forceinline__ void pair_weights(const PointGeom& pc, const Texel& g, float vis,
                                              float& w1, float& w2) {
   const Lobe lb = eval_lobe<SPEC>(pc, g);
   w1 = lb.nl * vis;
@@ -110,17 +87,16 @@ __device__ __forceinline__ void pair_weights(const PointC& pc, const float4& g, 
 
 // ---- preparation kernels ----------------------------------------------------------
 
-// texel table, zero beyond T (tp entries)
+// texel table, zero beyond T (tp entries); the radiance fields are unused here
 __global__ __launch_bounds__(256) void relight_texels_kernel(RelightArgs a) {
   const int t = blockIdx.x * 256 + threadIdx.x;
   if (t >= a.tp) return;
-  float4 g = make_float4(0.f, 0.f, 0.f, 0.f);
+  Texel g = zero_texel();
   if (t < a.T) {
     double M[9];
 #pragma unroll
     for (int i = 0; i < 9; ++i) M[i] = (double)a.rotmat[i];
-    const TexelGeo tg = texel_geometry(t, a.H, a.W, M, solid_angle_scale(a.H, a.W));
-    g = make_float4(tg.lx, tg.ly, tg.lz, tg.ll);
+    g = make_texel(texel_geometry(t, a.H, a.W, M, solid_angle_scale(a.H, a.W)), 0.f, 0.f, 0.f);
   }
   a.geo[t] = g;
 }
@@ -154,10 +130,7 @@ __global__ __launch_bounds__(64) void relight_L_kernel(RelightArgs a) {
 __global__ __launch_bounds__(256) void relight_points_kernel(RelightArgs a, bool spec) {
   const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (p >= (int64_t)a.kp * 32) return;
-  const PointC c = load_point_consts(a, p, spec);
-  a.pconst[3 * p + 0] = make_float4(c.nx, c.ny, c.nz, c.nv);
-  a.pconst[3 * p + 1] = make_float4(c.vx, c.vy, c.vz, c.vv);
-  a.pconst[3 * p + 2] = make_float4(c.k, c.ib, c.a2, 0.f);
+  a.pconst[p] = load_point_consts(a, p, spec);
 }
 
 // A / B fragments of the env-gradient GEMM: block (k-step ks, n-tile n), lane l,
@@ -198,11 +171,11 @@ template <int PREC, bool SPEC>
 __global__ __launch_bounds__(256) void relight_forward_kernel(RelightArgs a) {
   constexpr int NF = (PREC == kPrecBF16x3) ? 2 : 1;
   __shared__ uint4 s_frag[kRelightMaxNT * NF * 64];
-  __shared__ float4 s_geo[32];
+  __shared__ Texel s_geo[32];
   const int lane = threadIdx.x & 63;
   const int wv = threadIdx.x >> 6;
   const int64_t p = (int64_t)blockIdx.x * 64 + wv * 16 + (lane & 15);
-  const PointC pc = load_point_consts(a, p, SPEC);
+  const PointGeom pc = load_point_consts(a, p, SPEC);
   const bool valid = p < a.P;
   const int tb = 8 * (lane >> 4);
 
@@ -281,12 +254,12 @@ template <int PREC, bool SPEC>
 __global__ __launch_bounds__(256) void relight_envgrad_kernel(RelightArgs a) {
   constexpr int NF = (PREC == kPrecBF16x3) ? 2 : 1;
   __shared__ uint4 s_ab[kRelightMaxNT * 2 * NF * 64];
-  __shared__ float4 s_pc[32 * 3];
+  __shared__ PointGeom s_pc[32];
   __shared__ uint32_t s_vis[2][32];
   const int lane = threadIdx.x & 63;
   const int wv = threadIdx.x >> 6;
   const int t = (blockIdx.x * 4 + wv) * 16 + (lane & 15);  // < tp
-  const float4 g = a.geo[t];
+  const Texel g = a.geo[t];
   const int slot_local = (t >> 5) - 2 * (int)blockIdx.x;    // 0 or 1
   const int tbit = t & 31;
   const int pb = 8 * (lane >> 4);
@@ -302,7 +275,7 @@ __global__ __launch_bounds__(256) void relight_envgrad_kernel(RelightArgs a) {
     __syncthreads();
     const uint4* src = a.abfrag + (int64_t)ks * nab;
     for (int i = threadIdx.x; i < nab; i += 256) s_ab[i] = src[i];
-    if (threadIdx.x < 96) s_pc[threadIdx.x] = a.pconst[(int64_t)ks * 96 + threadIdx.x];
+    if (threadIdx.x < 32) s_pc[threadIdx.x] = a.pconst[(int64_t)ks * 32 + threadIdx.x];
     if (threadIdx.x >= 128 && threadIdx.x < 192) {
       const int q = threadIdx.x - 128;  // (slot, point)
       const int sl = q >> 5, pt = q & 31;
@@ -317,11 +290,7 @@ __global__ __launch_bounds__(256) void relight_envgrad_kernel(RelightArgs a) {
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
       const int pt = pb + j;
-      const float4 q0 = s_pc[3 * pt], q1 = s_pc[3 * pt + 1], q2 = s_pc[3 * pt + 2];
-      PointC pc;
-      pc.nx = q0.x; pc.ny = q0.y; pc.nz = q0.z; pc.nv = q0.w;
-      pc.vx = q1.x; pc.vy = q1.y; pc.vz = q1.z; pc.vv = q1.w;
-      pc.k = q2.x; pc.omk = 1.f - q2.x; pc.ib = q2.y; pc.a2 = q2.z; pc.am1 = q2.z - 1.f;
+      const PointGeom& pc = s_pc[pt];
       const float vis = (float)__builtin_amdgcn_ubfe(s_vis[slot_local][pt], tbit, 1);
       pair_weights<SPEC>(pc, g, vis, w1[j], w2[j]);
     }

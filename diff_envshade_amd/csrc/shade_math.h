@@ -28,12 +28,21 @@ __device__ __forceinline__ double solid_angle_scale(int H, int W) {
   return 4.0 * kPiD * sinpi(0.5 / (double)H) / (double)W;
 }
 
-// Storage texel t -> rotated direction l = M d (fp32, rounded once from FP64),
-// |l|^2 and dOmega (FP64).  Row r = t / W, column x = t % W, flipped row
-// iy = H-1-r; theta = ((iy+.5)/H - .5) pi, phi = 2 pi (x+.5)/W,
-// d = (cos th cos ph, sin th, cos th sin ph).
+// One texel as the shading loops read it (48 bytes, 16-byte aligned).
+//   d*: rotated light direction l = M d in FP64;  l*: the same rounded to fp32;
+//   e*: E_c * dOmega (fp32, rounded once).
+struct Texel {
+  double dx, dy, dz;
+  float lx, ly, lz;
+  float e0, e1, e2;
+};
+static_assert(sizeof(Texel) == 48, "texel entry layout");
+
+// Storage texel t -> rotated direction and solid angle (FP64).  Row
+// r = t / W, column x = t % W, flipped row iy = H-1-r; theta = ((iy+.5)/H - .5) pi,
+// phi = 2 pi (x+.5)/W, d = (cos th cos ph, sin th, cos th sin ph), l = M d.
 struct TexelGeo {
-  float lx, ly, lz, ll;
+  double lx, ly, lz;
   double dw;
 };
 
@@ -47,12 +56,29 @@ __device__ __forceinline__ TexelGeo texel_geometry(int t, int H, int W, const do
   sincospi((2.0 * (double)x + 1.0) / (double)W, &sph, &cph);
   const double dx = cth * cph, dy = sth, dz = cth * sph;
   TexelGeo g;
-  g.lx = (float)(M[0] * dx + M[1] * dy + M[2] * dz);
-  g.ly = (float)(M[3] * dx + M[4] * dy + M[5] * dz);
-  g.lz = (float)(M[6] * dx + M[7] * dy + M[8] * dz);
-  g.ll = (float)((double)g.lx * g.lx + (double)g.ly * g.ly + (double)g.lz * g.lz);
+  g.lx = M[0] * dx + M[1] * dy + M[2] * dz;
+  g.ly = M[3] * dx + M[4] * dy + M[5] * dz;
+  g.lz = M[6] * dx + M[7] * dy + M[8] * dz;
   g.dw = cth * kdw;
   return g;
+}
+
+__device__ __forceinline__ Texel make_texel(const TexelGeo& g, float E0, float E1, float E2) {
+  Texel o;
+  o.dx = g.lx; o.dy = g.ly; o.dz = g.lz;
+  o.lx = (float)g.lx; o.ly = (float)g.ly; o.lz = (float)g.lz;
+  o.e0 = (float)((double)E0 * g.dw);
+  o.e1 = (float)((double)E1 * g.dw);
+  o.e2 = (float)((double)E2 * g.dw);
+  return o;
+}
+
+__device__ __forceinline__ Texel zero_texel() {
+  Texel o;
+  o.dx = o.dy = o.dz = 0.0;
+  o.lx = o.ly = o.lz = 0.f;
+  o.e0 = o.e1 = o.e2 = 0.f;
+  return o;
 }
 
 // ---- the per-pair lobe -------------------------------------------------------
@@ -61,7 +87,8 @@ __device__ __forceinline__ TexelGeo texel_geometry(int t, int H, int W, const do
 struct Lobe {
   float cs;    // l.n
   float nl;    // max(cs, 0)
-  float hx, hy, hz;  // l + v, formed componentwise (exact for near-opposite l, v)
+  float hx, hy, hz;  // hb = l + v (formed in FP64)
+  float cx, cy, cz;  // n x hb (formed in FP64)
   float rs;    // 1/|l+v|
   float ch;    // l.h
   float nh;    // n.h
@@ -74,43 +101,89 @@ struct Lobe {
   bool spec_on;
 };
 
-// S provides nx..nz, vx..vz, k, omk, ib, a2, am1 (per-point constants).
+// S provides nx..nz, vx..vz (fp32), nxd..nzd, vxd..vzd (the same in FP64),
+// cn = 1 - |n|^2 (from FP64), k, omk, ib, a2, am1 (per-point constants).
+//
+// Geometry of the half vector in FP64 (v_fma_f64 runs at the fp32 vector rate
+// on gfx950): hb = l + v, |hb|^2, n.hb and |n x hb|^2 are formed exactly
+// enough that the NDF denominator
+//   tD = (n.h)^2 (alpha^2 - 1) + 1 = alpha^2 (n.h)^2 + (1 - (n.h)^2),
+//   1 - (n.h)^2 = (|n x hb|^2 + (1 - |n|^2) |hb|^2) / |hb|^2   (Lagrange identity)
+// keeps full fp32 relative accuracy at the peak of sharp lobes, where the
+// reference's fp32 form cancels (1 - (n.h)^2 ~ alpha^2 ~ 1e-4).
 template <bool SPEC, class S>
-__device__ __forceinline__ Lobe eval_lobe(const S& s, const float4& g) {
+__device__ __forceinline__ Lobe eval_lobe(const S& s, const Texel& g) {
   Lobe o;
-  o.cs = fmaf(g.z, s.nz, fmaf(g.y, s.ny, g.x * s.nx));
+  o.cs = fmaf(g.lz, s.nz, fmaf(g.ly, s.ny, g.lx * s.nx));
   o.nl = fmaxf(o.cs, 0.f);
   if (SPEC) {
-    // h = (l+v)/|l+v| (bs:2218-2219); l+v componentwise keeps |l+v|, l.h and
-    // n.h accurate to fp32 rounding when l and v are nearly opposite
-    o.hx = g.x + s.vx;
-    o.hy = g.y + s.vy;
-    o.hz = g.z + s.vz;
-    const float hb2 = fmaf(o.hz, o.hz, fmaf(o.hy, o.hy, o.hx * o.hx));
+    // h = (l+v)/|l+v| (bs:2218-2219)
+    const double hx = g.dx + s.vxd, hy = g.dy + s.vyd, hz = g.dz + s.vzd;
+    const double hb2 = fma(hz, hz, fma(hy, hy, hx * hx));
+    const double nhb = fma(s.nzd, hz, fma(s.nyd, hy, s.nxd * hx));
+    const double cx = fma(s.nyd, hz, -s.nzd * hy);
+    const double cy = fma(s.nzd, hx, -s.nxd * hz);
+    const double cz = fma(s.nxd, hy, -s.nyd * hx);
+    const float c2 = (float)fma(cz, cz, fma(cy, cy, cx * cx));
+    o.cx = (float)cx;
+    o.cy = (float)cy;
+    o.cz = (float)cz;
+    o.hx = (float)hx;
+    o.hy = (float)hy;
+    o.hz = (float)hz;
+    const float hb2f = (float)hb2;
     // floor keeps (n.h)^2 finite when v = -l exactly; such pairs are masked below
-    o.rs = rsq(fmaxf(hb2, 1e-20f));
-    o.ch = fmaf(g.z, o.hz, fmaf(g.y, o.hy, g.x * o.hx)) * o.rs;
-    o.nh = fmaf(s.nz, o.hz, fmaf(s.ny, o.hy, s.nx * o.hx)) * o.rs;
+    o.rs = rsq(fmaxf(hb2f, 1e-20f));
+    const float rs2 = o.rs * o.rs;
+    o.ch = fmaf(g.lz, o.hz, fmaf(g.ly, o.hy, g.lx * o.hx)) * o.rs;
+    o.nh = (float)nhb * o.rs;
     const float om = 1.f - sat(o.ch);
     const float om2 = om * om;
     o.om4 = om2 * om2;
     o.F = fmaf(o.om4 * om, 1.f - kF0, kF0);
     o.ia = rcp(fmaf(o.nl, s.omk, s.k));
     o.G = o.nl * o.ia * s.ib;
-    o.tD = fmaf(o.nh * o.nh, s.am1, 1.f);
+    o.tD = fmaf(o.nh * o.nh, s.a2, fmaf(s.cn, hb2f, c2) * rs2);
     o.it2 = rcp(fmaxf(kPi * o.tD * o.tD, kThres));
     o.D = s.a2 * o.it2;
     const float tmp = o.F * o.G * o.D * 0.25f;
     // the reference skips the lobe unless tmp > 0; |l+v| = 0 gives NaN there
-    o.spec_on = (tmp > 0.f) && (hb2 > 0.f);
+    o.spec_on = (tmp > 0.f) && (hb2f > 0.f);
     o.tmp = o.spec_on ? tmp : 0.f;
   } else {
     o.hx = o.hy = o.hz = 0.f;
+    o.cx = o.cy = o.cz = 0.f;
     o.rs = o.ch = o.nh = o.om4 = o.F = o.G = o.D = o.ia = o.it2 = o.tD = 0.f;
     o.spec_on = false;
     o.tmp = 0.f;
   }
   return o;
+}
+
+// Per-point constants in the form eval_lobe reads them.
+struct PointGeom {
+  float nx, ny, nz, vx, vy, vz;
+  double nxd, nyd, nzd, vxd, vyd, vzd;
+  float nv, cn, k, omk, ib, a2, am1;
+};
+
+__device__ __forceinline__ PointGeom point_geom(float nx, float ny, float nz, float vx, float vy,
+                                                float vz, float r) {
+  PointGeom c;
+  c.nx = nx; c.ny = ny; c.nz = nz;
+  c.vx = vx; c.vy = vy; c.vz = vz;
+  c.nxd = nx; c.nyd = ny; c.nzd = nz;
+  c.vxd = vx; c.vyd = vy; c.vzd = vz;
+  c.nv = nx * vx + ny * vy + nz * vz;
+  c.cn = (float)(1.0 - fma(c.nzd, c.nzd, fma(c.nyd, c.nyd, c.nxd * c.nxd)));
+  // bs:2126-2129: alpha2 = r^4, k = (r+1)^2 / 8
+  const float r2 = r * r;
+  c.a2 = r2 * r2;
+  c.am1 = c.a2 - 1.f;
+  c.k = (r + 1.f) * (r + 1.f) * 0.125f;
+  c.omk = 1.f - c.k;
+  c.ib = 1.f / (fmaxf(c.nv, 0.f) * c.omk + c.k);
+  return c;
 }
 
 }  // namespace math

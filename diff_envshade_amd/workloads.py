@@ -162,3 +162,59 @@ def dropin_headline(seed: int = 0, n: int = 70000, H: int = 16, W: int = 32) -> 
     d.update(identity_mesh(n))
     d.update(upstream_grads(rng, n))
     return d
+
+
+def make_visibility_device(normal, H: int, W: int, rotmat=None, p: float = 0.7, seed: int = 0,
+                           chunk: int = 32768):
+    """Device-side twin of make_visibility for large point sets (torch):
+    Bernoulli(p) AND upper hemisphere, packed [N_slot, N] int32 words.  Uses
+    torch's seeded generator, so the bits differ from make_visibility's."""
+    import torch
+    dev = normal.device
+    l, _ = texel_dirs(H, W, rotmat)
+    lt = torch.from_numpy(l).to(dev, torch.float32)
+    n = normal.shape[0]
+    T = H * W
+    ns = (T + 31) // 32
+    out = torch.empty((ns, n), dtype=torch.int32, device=dev)
+    g = torch.Generator(device=dev)
+    g.manual_seed(seed)
+    weights = (torch.ones(32, dtype=torch.int64, device=dev) << torch.arange(32, device=dev))
+    for s in range(0, n, chunk):
+        e = min(n, s + chunk)
+        up = (normal[s:e].to(torch.float32) @ lt.T) > 0
+        keep = torch.rand((e - s, T), generator=g, device=dev) < p
+        bits = torch.zeros((e - s, ns * 32), dtype=torch.int64, device=dev)
+        bits[:, :T] = (up & keep).to(torch.int64)
+        words = (bits.view(e - s, ns, 32) * weights).sum(-1)
+        words = torch.where(words >= 2 ** 31, words - 2 ** 32, words)
+        out[:, s:e] = words.to(torch.int32).T
+    return out
+
+
+def log_uniform_sizes(rng: np.random.Generator, n: int, lo: int, hi: int) -> np.ndarray:
+    return np.rint(np.exp(rng.uniform(math.log(lo), math.log(hi), size=n))).astype(int)
+
+
+def uneven_batch(seed: int = 0, n_sets: int = 48, n_min: int = 2000, n_max: int = 300000,
+                 device="cuda"):
+    """Uneven batch: point sets with N log-uniform in [n_min, n_max], each with
+    its own 16x32 or 32x64 (p = 0.5) lognormal(0,1) env map, visibility as in
+    the drop-in headline (per-point words through the identity mesh), and its
+    own materials and upstream gradients.  Returns a list of dicts of device
+    tensors."""
+    import torch
+    rng = np.random.default_rng(seed)
+    sizes = log_uniform_sizes(rng, n_sets, n_min, n_max)
+    sets = []
+    for i, n in enumerate(sizes):
+        H, W = (16, 32) if rng.random() < 0.5 else (32, 64)
+        d = make_points(rng, int(n))
+        d["envmap"] = make_envmap(rng, H, W, 1.0)
+        d["light2obj_rotmat"] = np.eye(3, dtype=np.float32)
+        d.update(identity_mesh(int(n)))
+        d.update(upstream_grads(rng, int(n)))
+        t = {k: torch.from_numpy(v).to(device) for k, v in d.items()}
+        t["visibility"] = make_visibility_device(t["normal"], H, W, seed=seed * 1000 + i)
+        sets.append(t)
+    return sets

@@ -236,7 +236,7 @@ enum {
 DES_API int des_batch_plan(int32_t variant, const des_set* sets, int32_t n_sets,
                            size_t* workspace_bytes, size_t* staging_bytes);
 
-/* ntc_forward / ntc_backward: waves per 64-point tile (powers of two, <= 16 / <= 8),
+/* ntc_forward / ntc_backward: waves per 64-point tile (powers of two, <= 8),
  * usually from the autotuner; they must match the later launches.  with_backward:
  * the sets carry upstream gradients and gradient outputs. */
 DES_API int des_batch_prepare(int32_t variant, int enable_diffuse, int enable_specular,

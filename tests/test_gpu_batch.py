@@ -30,7 +30,7 @@ def _np(outs, variant="vis"):
 SPECS = [(300, 16, 32), (5000, 32, 64), (1, 4, 8), (0, 16, 32), (2049, 16, 32), (777, 8, 16)]
 
 
-@pytest.mark.parametrize("ntc_f,ntc_b", [(8, 8), (4, 2), (16, 1)])
+@pytest.mark.parametrize("ntc_f,ntc_b", [(8, 8), (4, 2), (1, 1)])
 def test_batch_matches_oracle_and_single_set(ntc_f, ntc_b):
     from diff_envshade_amd import _lib
     from diff_envshade_amd.batch import ShadeBatch
@@ -123,7 +123,7 @@ def test_autotuner_picks_valid_shapes():
     nps, devs = _sets([(20000, 16, 32), (9000, 32, 64)], seed=70)
     b = ShadeBatch("vis")
     f, bw = tune_batch(b, devs, reps=2, force=True)
-    assert f in (1, 2, 4, 8, 16) and bw in (1, 2, 4, 8)
+    assert f in (1, 2, 4, 8) and bw in (1, 2, 4, 8)
     b.forward()
     b.backward()
     torch.cuda.synchronize()
