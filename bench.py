@@ -31,6 +31,23 @@ FLOP_BWD_PER_PAIR = 195
 PEAK_FP32_FLOPS = 157.3e12   # MI355X FP32 vector peak
 PEAK_HBM_BYTES = 8.0e12      # MI355X HBM3E
 
+# HBM bytes per launch of the dominant kernel (the backward) on the headline
+# workload, from two rocprofv3 --pmc runs (FETCH_SIZE, then WRITE_SIZE, KiB per
+# dispatch), each scaled by what the same counter reads for a plain 256 MiB
+# device copy (`bench.py --workload copy`): FETCH_SIZE x 2.000, WRITE_SIZE
+# x 1.000.  Commands and per-kernel figures:
+# profiles/hbm_traffic_dropin_70k_16x32.txt.  Reported only for the workload
+# the counters were read on.
+HEADLINE_TRAFFIC = {"points": 70000, "texels": 512, "fetch_kib": 6704.4, "fetch_factor": 2.0,
+                    "write_kib": 10118.7, "write_factor": 1.0}
+
+
+def _traffic(P, T):
+    t = HEADLINE_TRAFFIC
+    if (P, T) != (t["points"], t["texels"]):
+        return None
+    return (t["fetch_kib"] * t["fetch_factor"] + t["write_kib"] * t["write_factor"]) * 1024.0
+
 
 def _percentile(xs, q):
     xs = sorted(xs)
@@ -178,7 +195,8 @@ def main() -> int:
             "frac": achieved / PEAK_FP32_FLOPS,
             "flop_per_pair": FLOP_BWD_PER_PAIR,
             "pairs_per_launch": pairs,
-            "traffic": None,
+            "traffic": _traffic(P, T),
+            "traffic_unit": "bytes per launch (HBM, rocprofv3 FETCH_SIZE + WRITE_SIZE)",
         },
         "forward_roofline": {
             "achieved": pairs * FLOP_FWD_PER_PAIR / (fwd_avg_ms * 1e-3) / 1e12,
