@@ -346,7 +346,30 @@ def bench_relight(args):
     gd = torch.randn((P, E, 3), device=dev, generator=torch.Generator(dev).manual_seed(3))
     mat = (t["normal"], t["view_dir"], t["albedo"], t["specular_albedo"], t["roughness"])
     res = {}
-    O = None if args.no_accuracy else _oracle()
+    # FP64 references, computed once and shared by the three precisions: the
+    # forward on the first 256 points and the env-map gradients of the first
+    # 512 points, for every 8th map (8 of 64), all maps of the same problem.
+    n1, n2, maps = 256, 512, list(range(0, E, 8))
+    ref = None
+    if not args.no_accuracy:
+        O = _oracle()
+        base = {k: v.cpu().numpy() for k, v in t.items()}
+        visn = vis.cpu().numpy()
+        m1, m2 = WL.identity_mesh(n1), WL.identity_mesh(n2)
+        ref = {"sh": {}, "ds": {}, "gE": {}}
+        print(f"[bench] relight: FP64 references for {len(maps)} maps ...", file=sys.stderr,
+              flush=True)
+        for e in maps:
+            env_e = envs[e].cpu().numpy()
+            d1 = {k: v[:n1] for k, v in base.items()}
+            d1.update(m1, envmap=env_e, light2obj_rotmat=np.eye(3, dtype=np.float32),
+                      visibility=np.ascontiguousarray(visn[:, :n1]))
+            ref["sh"][e], ref["ds"][e] = O.forward("vis", d1)
+            d2 = {k: v[:n2] for k, v in base.items()}
+            d2.update(m2, envmap=env_e, light2obj_rotmat=np.eye(3, dtype=np.float32),
+                      visibility=np.ascontiguousarray(visn[:, :n2]))
+            ref["gE"][e] = O.backward("vis", d2, gs[:n2, e].cpu().numpy(),
+                                      gd[:n2, e].cpu().numpy())["envmap"]
     for prec in ("bf16", "fp16", "bf16x3"):
         ws_f = torch.empty(workspace_size(P, E, H, W, prec), dtype=torch.uint8, device=dev)
         ws_b = torch.empty(workspace_size(P, E, H, W, prec, True), dtype=torch.uint8, device=dev)
@@ -359,42 +382,26 @@ def bench_relight(args):
              "pairs_per_s": P * E / ((ms_f + ms_b) * 1e-3)}
         print(f"[bench] relight {prec}: fwd {ms_f:.3f} ms, envgrad {ms_b:.3f} ms",
               file=sys.stderr, flush=True)
-        if O is not None:
-            # accuracy on a 256-point subset (forward) and a 2048-point problem (env grads)
-            n = 256
-            sub = {k: v[:n].contiguous() for k, v in t.items()}
-            sv = vis[:, :n].contiguous()
-            sh, ds = relight_forward(sub["normal"], sub["view_dir"], sub["albedo"],
-                                     sub["specular_albedo"], sub["roughness"], envs, rot, sv, True,
-                                     prec)
-            n2 = 2048
-            sub2 = {k: v[:n2].contiguous() for k, v in t.items()}
-            sv2 = vis[:, :n2].contiguous()
-            gE = relight_backward_envmap(sub2["normal"], sub2["view_dir"], sub2["albedo"],
-                                         sub2["specular_albedo"], sub2["roughness"], H, W, rot,
-                                         sv2, gs[:n2].contiguous(), gd[:n2].contiguous(), True,
-                                         prec)
-            errs = {"shading": 0.0, "diffuse_shading": 0.0, "grad_envmap": 0.0}
+        if ref is not None:
+            sub = [v[:n1].contiguous() for v in mat]
+            sh, ds = relight_forward(*sub, envs, rot, vis[:, :n1].contiguous(), True, prec)
+            sub2 = [v[:n2].contiguous() for v in mat]
+            gE = relight_backward_envmap(*sub2, H, W, rot, vis[:, :n2].contiguous(),
+                                         gs[:n2].contiguous(), gd[:n2].contiguous(), True, prec)
             shn, dsn, gEn = sh.cpu().numpy(), ds.cpu().numpy(), gE.cpu().numpy()
-            base = {k: v.cpu().numpy() for k, v in sub.items()}
-            base2 = {k: v.cpu().numpy() for k, v in sub2.items()}
-            m1, m2 = WL.identity_mesh(n), WL.identity_mesh(n2)
-            num = {k: 0.0 for k in errs}
-            den = {k: 0.0 for k in errs}
-            for e in range(E):
-                de = dict(base, **m1, envmap=envs[e].cpu().numpy(), light2obj_rotmat=np.eye(3),
-                          visibility=sv.cpu().numpy())
-                rs, rd = O.forward("vis", de)
-                num["shading"] = max(num["shading"], np.abs(shn[:, e] - rs).max())
-                den["shading"] = max(den["shading"], np.abs(rs).max())
-                num["diffuse_shading"] = max(num["diffuse_shading"], np.abs(dsn[:, e] - rd).max())
-                den["diffuse_shading"] = max(den["diffuse_shading"], np.abs(rd).max())
-                de2 = dict(base2, **m2, envmap=envs[e].cpu().numpy(), light2obj_rotmat=np.eye(3),
-                           visibility=sv2.cpu().numpy())
-                g = O.backward("vis", de2, gs[:n2, e].cpu().numpy(), gd[:n2, e].cpu().numpy())
-                num["grad_envmap"] = max(num["grad_envmap"], np.abs(gEn[e] - g["envmap"]).max())
-                den["grad_envmap"] = max(den["grad_envmap"], np.abs(g["envmap"]).max())
-            r["err_vs_fp64_max_rel"] = {k: num[k] / den[k] for k in num}
+            err = {}
+            for name, got in (("shading", lambda e: shn[:, e]), ("diffuse_shading",
+                                                                 lambda e: dsn[:, e]),
+                              ("grad_envmap", lambda e: gEn[e])):
+                key = {"shading": "sh", "diffuse_shading": "ds", "grad_envmap": "gE"}[name]
+                num = max(float(np.abs(got(e) - ref[key][e]).max()) for e in maps)
+                den = max(float(np.abs(ref[key][e]).max()) for e in maps)
+                mnum = np.mean([float(np.abs(got(e) - ref[key][e]).mean()) for e in maps])
+                mden = np.mean([float(np.abs(ref[key][e]).mean()) for e in maps])
+                err[name] = {"max_rel": num / den, "mean_rel": float(mnum / mden)}
+            r["err_vs_fp64"] = err
+            r["err_sample"] = (f"forward: {n1} points, env-map gradients: {n2} points; maps "
+                               f"{maps[0]}..{maps[-1]} step 8; normwise max|x-ref|/max|ref|")
         res[prec] = r
         del ws_f, ws_b
     # the fp32 drop-in on the same points and the first map, forward + backward
@@ -413,12 +420,17 @@ def bench_relight(args):
                                                          mesh["barycentric_coord"], vis, True)
     ms_d, _ = _time_ms(dropin, reps=5)
     dropin_rate = P / (ms_d * 1e-3)
-    best = res["bf16"]["pairs_per_s"]
+    # headline precision: the fastest whose every output is within 1e-3 of
+    # FP64 (all of them when accuracy was not measured)
+    ok = [k for k, v in res.items()
+          if "err_vs_fp64" not in v or max(e["max_rel"] for e in v["err_vs_fp64"].values()) <= 1e-3]
+    best = max(ok or list(res), key=lambda k: res[k]["pairs_per_s"])
     return {
         "metric": "point·envmap pairs/s, forward + env-map gradients, 64 maps on MFMA",
-        "value": best, "unit": "point·envmap pairs/s", "n_gpus": 1,
-        "ms_per_step": res["bf16"]["forward_ms"] + res["bf16"]["envgrad_ms"],
-        "higher_is_better": True, "dtype": "bf16 operands, fp32 accumulate", "data": "synthetic",
+        "value": res[best]["pairs_per_s"], "unit": "point·envmap pairs/s", "n_gpus": 1,
+        "ms_per_step": res[best]["forward_ms"] + res[best]["envgrad_ms"],
+        "higher_is_better": True, "dtype": f"{best} operands, fp32 accumulate",
+        "precision": best, "data": "synthetic",
         "config": {"workload": "262,144 points with 2048-bit visibility x 64 env maps 32x64 "
                                "lognormal(0,1.5): forward + env-map gradients",
                    "points": P, "maps": E, "env_h": H, "env_w": W},
@@ -441,7 +453,7 @@ def bench_serving(args):
     shapes = [(16, 32), (32, 64)] * 16
     pool = DevicePool(max_points=300000, env_shapes=shapes, seed=0)
     # capacity: one batch of 48 requests drawn from the same distribution
-    probe = PoissonWorkload(n_requests=48, rate=1e9, n_env=32, seed=11)
+    probe = PoissonWorkload(n_requests=48, rate=1e9, n_env=32, fwd_only=0.0, seed=11)
     preqs = probe.requests(lambda r, e: pool.request_payload(r, e, 0))
     b = ShadeBatch("vis")
     fw = [r.payload for r in preqs]

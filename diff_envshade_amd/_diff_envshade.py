@@ -111,6 +111,15 @@ def _workspace(P: int, H: int, W: int, dev: torch.device):
     return torch.empty(n, dtype=torch.uint8, device=dev), n
 
 
+def _fwd_workspace(P: int, H: int, W: int, dev: torch.device):
+    """Point-order workspace of the forward entry points (None when the
+    library would not reorder this problem)."""
+    n = _lib.forward_workspace_size(P, H, W)
+    if n == 0:
+        return None, 0
+    return torch.empty(n, dtype=torch.uint8, device=dev), n
+
+
 def _material_checks(P, C, normal, view_dir, albedo, specular_albedo, roughness, envmap,
                      light2obj_rotmat):
     for name, t in (("normal", normal), ("view_dir", view_dir), ("albedo", albedo),
@@ -165,8 +174,10 @@ def bruteforce_diffuse_shader_forward(normal, albedo, envmap, light2obj_rotmat):
                        ("light2obj_rotmat", light2obj_rotmat)))
     shading = torch.empty((P, C), dtype=normal.dtype, device=dev)
     with torch.cuda.device(dev):
+        ws, nws = _fwd_workspace(P, H, W, dev)
         _lib.call("des_bruteforce_diffuse_shader_forward", P, C, H, W, _ptr(normal), _ptr(albedo),
-                  _ptr(envmap), _ptr(light2obj_rotmat), _ptr(shading), _stream(dev))
+                  _ptr(envmap), _ptr(light2obj_rotmat), _ptr(shading), _ptr(ws), nws,
+                  _stream(dev))
     return shading
 
 
@@ -205,10 +216,11 @@ def _spec_forward(entry, normal, view_dir, albedo, specular_albedo, roughness, e
     dev = normal.device
     shading = torch.empty((P, C), dtype=normal.dtype, device=dev)
     with torch.cuda.device(dev):
+        ws, nws = _fwd_workspace(P, H, W, dev)
         _lib.call(entry, P, C, H, W, _ptr(normal), _ptr(view_dir), _ptr(albedo),
                   _ptr(specular_albedo), _ptr(roughness), _ptr(envmap), _ptr(light2obj_rotmat),
                   int(bool(enable_diffuse)), int(bool(enable_specular)), _ptr(shading),
-                  _stream(dev))
+                  _ptr(ws), nws, _stream(dev))
     return shading
 
 
@@ -287,10 +299,11 @@ def bruteforce_specular_shader_forward2(normal, view_dir, albedo, specular_albed
     shading = torch.empty((P, C), dtype=normal.dtype, device=dev)
     diffuse_shading = torch.empty((P, C), dtype=normal.dtype, device=dev)
     with torch.cuda.device(dev):
+        ws, nws = _fwd_workspace(P, H, W, dev)
         _lib.call("des_bruteforce_specular_shader_forward2", P, C, H, W, _ptr(normal),
                   _ptr(view_dir), _ptr(albedo), _ptr(specular_albedo), _ptr(roughness),
                   _ptr(envmap), _ptr(light2obj_rotmat), int(bool(enable_specular)),
-                  _ptr(shading), _ptr(diffuse_shading), _stream(dev))
+                  _ptr(shading), _ptr(diffuse_shading), _ptr(ws), nws, _stream(dev))
     return [shading, diffuse_shading]
 
 
@@ -336,12 +349,13 @@ def bruteforce_specularvisibility_shader_forward2(normal, view_dir, albedo, spec
     shading = torch.empty((P, C), dtype=normal.dtype, device=dev)
     diffuse_shading = torch.empty((P, C), dtype=normal.dtype, device=dev)
     with torch.cuda.device(dev):
+        ws, nws = _fwd_workspace(P, H, W, dev)
         _lib.call("des_bruteforce_specularvisibility_shader_forward2", P, C, H, W, NF, NV, NS,
                   _ptr(normal), _ptr(view_dir), _ptr(albedo), _ptr(specular_albedo),
                   _ptr(roughness), _ptr(envmap), _ptr(light2obj_rotmat), _ptr(face_vertex_list),
                   _ptr(nearest_triangle_id), _ptr(barycentric_coord), _ptr(visibility),
-                  int(bool(enable_specular)), _ptr(shading), _ptr(diffuse_shading),
-                  _stream(dev))
+                  int(bool(enable_specular)), _ptr(shading), _ptr(diffuse_shading), _ptr(ws),
+                  nws, _stream(dev))
     return [shading, diffuse_shading]
 
 

@@ -23,23 +23,24 @@ _sz = ctypes.c_size_t
 # argument kinds per entry point, in C order
 _SIGS = {
     "des_bruteforce_diffuse_shader_forward":
-        [_i64, _i32, _i32, _i32] + [_vp] * 4 + [_vp, _vp],
+        [_i64, _i32, _i32, _i32] + [_vp] * 4 + [_vp] + [_vp, _sz, _vp],
     "des_bruteforce_diffuse_shader_backward":
         [_i64, _i32, _i32, _i32] + [_vp] * 5 + [_vp] * 3 + [_vp, _sz, _vp],
     "des_bruteforce_specular_shader_forward":
-        [_i64, _i32, _i32, _i32] + [_vp] * 7 + [_int, _int] + [_vp, _vp],
+        [_i64, _i32, _i32, _i32] + [_vp] * 7 + [_int, _int] + [_vp] + [_vp, _sz, _vp],
     "des_bruteforce_specular_shader_backward":
         [_i64, _i32, _i32, _i32] + [_vp] * 8 + [_int, _int] + [_vp] * 6 + [_vp, _sz, _vp],
     "des_bruteforce_specular_shader_clamp_forward":
-        [_i64, _i32, _i32, _i32] + [_vp] * 7 + [_int, _int] + [_vp, _vp],
+        [_i64, _i32, _i32, _i32] + [_vp] * 7 + [_int, _int] + [_vp] + [_vp, _sz, _vp],
     "des_bruteforce_specular_shader_clamp_backward":
         [_i64, _i32, _i32, _i32] + [_vp] * 8 + [_int, _int] + [_vp] * 6 + [_vp, _sz, _vp],
     "des_bruteforce_specular_shader_forward2":
-        [_i64, _i32, _i32, _i32] + [_vp] * 7 + [_int] + [_vp, _vp, _vp],
+        [_i64, _i32, _i32, _i32] + [_vp] * 7 + [_int] + [_vp, _vp] + [_vp, _sz, _vp],
     "des_bruteforce_specular_shader_backward2":
         [_i64, _i32, _i32, _i32] + [_vp] * 9 + [_int] + [_vp] * 6 + [_vp, _sz, _vp],
     "des_bruteforce_specularvisibility_shader_forward2":
-        [_i64, _i32, _i32, _i32, _i32, _i32, _i32] + [_vp] * 11 + [_int] + [_vp, _vp, _vp],
+        [_i64, _i32, _i32, _i32, _i32, _i32, _i32] + [_vp] * 11 + [_int] + [_vp, _vp]
+        + [_vp, _sz, _vp],
     "des_bruteforce_specularvisibility_shader_backward2":
         [_i64, _i32, _i32, _i32, _i32, _i32, _i32] + [_vp] * 13 + [_int] + [_vp] * 6
         + [_vp, _sz, _vp],
@@ -89,6 +90,10 @@ def load() -> ctypes.CDLL:
         L.des_last_error.restype = ctypes.c_char_p
         L.des_backward_workspace_size.argtypes = [_i64, _i32, _i32]
         L.des_backward_workspace_size.restype = _sz
+        L.des_forward_workspace_size.argtypes = [_i64, _i32, _i32]
+        L.des_forward_workspace_size.restype = _sz
+        L.des_set_point_order.argtypes = [_i32]
+        L.des_set_point_order.restype = _int
         L.des_set_tiling_override.argtypes = [_i32, _i32]
         L.des_set_tiling_override.restype = _int
         L.des_set_backward_timing_events.argtypes = [_vp, _vp]
@@ -109,6 +114,15 @@ def call(name: str, *args) -> None:
 
 def workspace_size(P: int, H: int, W: int) -> int:
     return int(load().des_backward_workspace_size(P, H, W))
+
+
+def forward_workspace_size(P: int, H: int, W: int) -> int:
+    return int(load().des_forward_workspace_size(P, H, W))
+
+
+def set_point_order(mode: int) -> None:
+    """0 automatic, 1 always, -1 never (see des_set_point_order)."""
+    check(load().des_set_point_order(mode))
 
 
 def set_backward_timing_events(start, stop) -> None:

@@ -19,7 +19,8 @@ LIB_DIR = os.path.join(PKG_DIR, "lib")
 LIB_NAME = "libdiffenvshade.so"
 LIB_PATH = os.path.join(LIB_DIR, LIB_NAME)
 
-SOURCES = ["envshade_kernels.hip", "envshade_capi.cpp", "relight_kernels.hip", "relight_capi.cpp"]
+SOURCES = ["envshade_kernels.hip", "point_order.hip", "envshade_capi.cpp", "relight_kernels.hip",
+           "relight_capi.cpp"]
 HEADERS = ["envshade_internal.h", os.path.join("..", "..", "include", "diff_envshade.h")]
 
 ARCH = "gfx950"

@@ -46,10 +46,14 @@ thread_local hipEvent_t g_bwd_ev_start = nullptr;
 thread_local hipEvent_t g_bwd_ev_stop = nullptr;
 std::atomic<int> g_ntc_fwd_override{0};
 std::atomic<int> g_ntc_bwd_override{0};
+std::atomic<int> g_order_mode{0};  // 0 automatic, 1 always, -1 never
 
 constexpr int kTileP = 64;
 constexpr int64_t kMaxTexels = int64_t(1) << 24;
-constexpr int kTargetWaves = 8192;  // ~8 waves per SIMD on 256 CUs x 4 SIMDs
+constexpr int kMinWaves = 1024;  // one wave per SIMD on 256 CUs x 4 SIMDs
+// Automatic point order from this many point-texel pairs up: below it the
+// three sort launches cost about what the skipped texels save.
+constexpr int64_t kOrderMinPairs = int64_t(1) << 24;
 
 int64_t n_tiles(int64_t P) { return (P + kTileP - 1) / kTileP; }
 
@@ -82,25 +86,47 @@ des::Tiling choose_tiling(int64_t P, int T, bool backward) {
   if (ov > 0) {
     ntc = std::min(ov, cap);
   } else {
-    while (ntc * 2 <= cap && ntc * 2 <= nslot && tiles * ntc < kTargetWaves) ntc *= 2;
+    // Measured on MI355X from 20k to 262k points: a tile runs best over 8
+    // waves in the forward and 4 in the backward (whose register use allows
+    // 3 waves per SIMD, i.e. three 4-wave tiles per CU).  Problems of fewer
+    // than one wave per SIMD at that split use more waves per tile.
+    const int base = backward ? 4 : 8;
+    while (ntc * 2 <= base && ntc * 2 <= nslot) ntc *= 2;
+    while (ntc * 2 <= cap && ntc * 2 <= nslot && tiles * ntc < kMinWaves) ntc *= 2;
   }
   return tiling_for(ntc, T, backward);
 }
 
 size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
+// Whether the single-problem entry points sort the points by normal direction.
+bool use_order(int64_t P, int T) {
+  if (P <= 0 || P > int64_t(0x7fffffff)) return false;  // the order is int32
+  const int mode = g_order_mode.load();
+  if (mode != 0) return mode > 0;
+  return P * int64_t(T) >= kOrderMinPairs;
+}
+
 struct WsLayout {
-  size_t part_off, part2_off, total;
+  size_t part_off, part2_off, order_off, total;
+  bool ordered;
 };
 
-WsLayout ws_layout(int64_t P, int T) {
-  WsLayout w{0, 0, 0};
+// Backward workspace: per-tile fp32 partials, fp64 group sums, and (when the
+// points are reordered) the point-order workspace.
+WsLayout ws_layout(int64_t P, int T, bool allow_order) {
+  WsLayout w{0, 0, 0, 0, false};
   if (P <= 0) return w;
   const int64_t tiles = n_tiles(P);
   const int64_t groups = (tiles + des::kReduceRows - 1) / des::kReduceRows;
   w.part_off = 0;
   w.part2_off = align256(size_t(tiles) * 3 * T * sizeof(float));
   w.total = w.part2_off + align256(size_t(groups) * 3 * T * sizeof(double));
+  if (allow_order && use_order(P, T)) {
+    w.ordered = true;
+    w.order_off = w.total;
+    w.total += des::order_layout(P).total;
+  }
   return w;
 }
 
@@ -137,8 +163,20 @@ int launch_error(hipError_t e, const char* what) {
   return fail(DES_ERR_LAUNCH, "%s: %s", what, hipGetErrorString(e));
 }
 
-int run_forward(int variant, bool diff, bool spec, des::Problem& pr, hipStream_t stream) {
+int run_forward(int variant, bool diff, bool spec, des::Problem& pr, void* ws, size_t ws_bytes,
+                hipStream_t stream) {
   if (pr.P == 0) return ok();
+  if (ws != nullptr && use_order(pr.P, pr.T)) {
+    const des::OrderLayout ol = des::order_layout(pr.P);
+    if (ws_bytes < ol.total)
+      return fail(DES_ERR_WORKSPACE_TOO_SMALL, "workspace of %zu bytes is smaller than the %zu required",
+                  ws_bytes, ol.total);
+    if ((reinterpret_cast<uintptr_t>(ws) & 255) != 0)
+      return fail(DES_ERR_INVALID_ARGUMENT, "workspace must be 256-byte aligned");
+    hipError_t e = des::launch_point_order(pr.normal, pr.P, ws, stream);
+    if (e != hipSuccess) return launch_error(e, "point order failed");
+    pr.order = reinterpret_cast<const int32_t*>(static_cast<char*>(ws) + ol.order_off);
+  }
   const des::Tiling tl = choose_tiling(pr.P, pr.T, false);
   hipError_t e = des::launch_forward(variant, diff, spec, pr, tl, stream);
   if (e != hipSuccess) return launch_error(e, "forward launch failed");
@@ -153,7 +191,7 @@ int run_backward(int variant, bool diff, bool spec, des::Problem& pr, void* ws, 
     if (e != hipSuccess) return launch_error(e, "zero fill failed");
     return ok();
   }
-  const WsLayout lay = ws_layout(pr.P, pr.T);
+  const WsLayout lay = ws_layout(pr.P, pr.T, true);
   if (ws == nullptr || ws_bytes < lay.total)
     return fail(DES_ERR_WORKSPACE_TOO_SMALL, "workspace of %zu bytes is smaller than the %zu required",
                 ws_bytes, lay.total);
@@ -161,6 +199,12 @@ int run_backward(int variant, bool diff, bool spec, des::Problem& pr, void* ws, 
     return fail(DES_ERR_INVALID_ARGUMENT, "workspace must be 256-byte aligned");
   pr.part = reinterpret_cast<float*>(static_cast<char*>(ws) + lay.part_off);
   pr.part2 = reinterpret_cast<double*>(static_cast<char*>(ws) + lay.part2_off);
+  if (lay.ordered) {
+    char* ows = static_cast<char*>(ws) + lay.order_off;
+    hipError_t e = des::launch_point_order(pr.normal, pr.P, ows, stream);
+    if (e != hipSuccess) return launch_error(e, "point order failed");
+    pr.order = reinterpret_cast<const int32_t*>(ows + des::order_layout(pr.P).order_off);
+  }
   const des::Tiling tl = choose_tiling(pr.P, pr.T, true);
   hipEvent_t ev0 = g_bwd_ev_start, ev1 = g_bwd_ev_stop;
   g_bwd_ev_start = g_bwd_ev_stop = nullptr;
@@ -207,7 +251,7 @@ BatchLayout batch_layout(const des_set* sets, int n) {
   L.part_off.resize(n);
   L.part2_off.resize(n);
   for (int i = 0; i < n; ++i) {
-    const WsLayout w = ws_layout(sets[i].P, sets[i].H * sets[i].W);
+    const WsLayout w = ws_layout(sets[i].P, sets[i].H * sets[i].W, false);
     L.part_off[i] = off + w.part_off;
     L.part2_off[i] = off + w.part2_off;
     off += w.total;
@@ -287,13 +331,24 @@ des::BatchLaunch batch_launch(void* ws, int variant, bool diff, bool spec, bool 
 
 extern "C" {
 
-DES_API int des_abi_version(void) { return 1; }
+DES_API int des_abi_version(void) { return 2; }
 
 DES_API const char* des_last_error(void) { return g_last_error.c_str(); }
 
 DES_API size_t des_backward_workspace_size(int64_t P, int32_t H, int32_t W) {
   if (P <= 0 || H <= 0 || W <= 0) return 0;
-  return ws_layout(P, H * W).total;
+  return ws_layout(P, H * W, true).total;
+}
+
+DES_API size_t des_forward_workspace_size(int64_t P, int32_t H, int32_t W) {
+  if (P <= 0 || H <= 0 || W <= 0 || !use_order(P, H * W)) return 0;
+  return des::order_layout(P).total;
+}
+
+DES_API int des_set_point_order(int32_t mode) {
+  if (mode < -1 || mode > 1) return fail(DES_ERR_INVALID_ARGUMENT, "mode must be -1, 0 or 1");
+  g_order_mode.store(mode);
+  return ok();
 }
 
 DES_API int des_set_tiling_override(int32_t ntc_forward, int32_t ntc_backward) {
@@ -318,6 +373,7 @@ DES_API int des_bruteforce_diffuse_shader_forward(int64_t P, int32_t C, int32_t 
                                                   const float* normal, const float* albedo,
                                                   const float* envmap,
                                                   const float* light2obj_rotmat, float* shading,
+                                                  void* workspace, size_t workspace_bytes,
                                                   void* stream) {
   if (int rc = check_common(P, C, H, W)) return rc;
   if (P > 0) {
@@ -326,7 +382,8 @@ DES_API int des_bruteforce_diffuse_shader_forward(int64_t P, int32_t C, int32_t 
   des::Problem pr = make_problem(P, H, W);
   pr.normal = normal; pr.albedo = albedo; pr.envmap = envmap; pr.rotmat = light2obj_rotmat;
   pr.sh = shading;
-  return run_forward(des::kDiffuse, true, false, pr, static_cast<hipStream_t>(stream));
+  return run_forward(des::kDiffuse, true, false, pr, workspace, workspace_bytes,
+                     static_cast<hipStream_t>(stream));
 }
 
 DES_API int des_bruteforce_diffuse_shader_backward(
@@ -351,7 +408,8 @@ static int specular_forward(int variant, int64_t P, int32_t C, int32_t H, int32_
                             const float* normal, const float* view_dir, const float* albedo,
                             const float* specular_albedo, const float* roughness,
                             const float* envmap, const float* light2obj_rotmat, bool diff,
-                            bool spec, float* shading, float* diffuse_shading, void* stream) {
+                            bool spec, float* shading, float* diffuse_shading, void* workspace,
+                            size_t workspace_bytes, void* stream) {
   if (int rc = check_common(P, C, H, W)) return rc;
   if (P > 0) {
     REQUIRE(normal); REQUIRE(envmap); REQUIRE(light2obj_rotmat); REQUIRE(shading);
@@ -364,7 +422,8 @@ static int specular_forward(int variant, int64_t P, int32_t C, int32_t H, int32_
   pr.normal = normal; pr.view_dir = view_dir; pr.albedo = albedo; pr.spec = specular_albedo;
   pr.rough = roughness; pr.envmap = envmap; pr.rotmat = light2obj_rotmat;
   pr.sh = shading; pr.ds = diffuse_shading;
-  return run_forward(variant, diff, spec, pr, static_cast<hipStream_t>(stream));
+  return run_forward(variant, diff, spec, pr, workspace, workspace_bytes,
+                     static_cast<hipStream_t>(stream));
 }
 
 static int specular_backward(int variant, int64_t P, int32_t C, int32_t H, int32_t W,
@@ -400,10 +459,11 @@ DES_API int des_bruteforce_specular_shader_forward(
     int64_t P, int32_t C, int32_t H, int32_t W, const float* normal, const float* view_dir,
     const float* albedo, const float* specular_albedo, const float* roughness,
     const float* envmap, const float* light2obj_rotmat, int enable_diffuse, int enable_specular,
-    float* shading, void* stream) {
+    float* shading, void* workspace, size_t workspace_bytes, void* stream) {
   return specular_forward(des::kSpecular, P, C, H, W, normal, view_dir, albedo, specular_albedo,
                           roughness, envmap, light2obj_rotmat, enable_diffuse != 0,
-                          enable_specular != 0, shading, nullptr, stream);
+                          enable_specular != 0, shading, nullptr, workspace, workspace_bytes,
+                          stream);
 }
 
 DES_API int des_bruteforce_specular_shader_backward(
@@ -424,10 +484,11 @@ DES_API int des_bruteforce_specular_shader_clamp_forward(
     int64_t P, int32_t C, int32_t H, int32_t W, const float* normal, const float* view_dir,
     const float* albedo, const float* specular_albedo, const float* roughness,
     const float* envmap, const float* light2obj_rotmat, int enable_diffuse, int enable_specular,
-    float* shading, void* stream) {
+    float* shading, void* workspace, size_t workspace_bytes, void* stream) {
   return specular_forward(des::kClamp, P, C, H, W, normal, view_dir, albedo, specular_albedo,
                           roughness, envmap, light2obj_rotmat, enable_diffuse != 0,
-                          enable_specular != 0, shading, nullptr, stream);
+                          enable_specular != 0, shading, nullptr, workspace, workspace_bytes,
+                          stream);
 }
 
 DES_API int des_bruteforce_specular_shader_clamp_backward(
@@ -448,10 +509,10 @@ DES_API int des_bruteforce_specular_shader_forward2(
     int64_t P, int32_t C, int32_t H, int32_t W, const float* normal, const float* view_dir,
     const float* albedo, const float* specular_albedo, const float* roughness,
     const float* envmap, const float* light2obj_rotmat, int enable_specular, float* shading,
-    float* diffuse_shading, void* stream) {
+    float* diffuse_shading, void* workspace, size_t workspace_bytes, void* stream) {
   return specular_forward(des::kV2, P, C, H, W, normal, view_dir, albedo, specular_albedo,
                           roughness, envmap, light2obj_rotmat, true, enable_specular != 0,
-                          shading, diffuse_shading, stream);
+                          shading, diffuse_shading, workspace, workspace_bytes, stream);
 }
 
 DES_API int des_bruteforce_specular_shader_backward2(
@@ -475,7 +536,7 @@ DES_API int des_bruteforce_specularvisibility_shader_forward2(
     const float* light2obj_rotmat, const int32_t* face_vertex_list,
     const int64_t* nearest_triangle_id, const float* barycentric_coord,
     const int32_t* visibility, int enable_specular, float* shading, float* diffuse_shading,
-    void* stream) {
+    void* workspace, size_t workspace_bytes, void* stream) {
   if (int rc = check_common(P, C, H, W)) return rc;
   if (int rc = check_vis(P, H * W, N_face, N_vertex, N_slot)) return rc;
   if (P > 0) {
@@ -491,7 +552,8 @@ DES_API int des_bruteforce_specularvisibility_shader_forward2(
   pr.fvl = face_vertex_list; pr.tri = nearest_triangle_id; pr.bary = barycentric_coord;
   pr.vis = reinterpret_cast<const uint32_t*>(visibility);
   pr.sh = shading; pr.ds = diffuse_shading;
-  return run_forward(des::kVis, true, enable_specular != 0, pr, static_cast<hipStream_t>(stream));
+  return run_forward(des::kVis, true, enable_specular != 0, pr, workspace, workspace_bytes,
+                     static_cast<hipStream_t>(stream));
 }
 
 DES_API int des_bruteforce_specularvisibility_shader_backward2(
@@ -589,11 +651,12 @@ DES_API int des_batch_prepare(int32_t variant, int enable_diffuse, int enable_sp
     b.tile0 = tile0;
     b.ngroups = int32_t((tiles + des::kReduceRows - 1) / des::kReduceRows);
     b.colblk = (3 * pr.T + 255) / 256;
+    b.colblk16 = (3 * pr.T + 15) / 16;
     b.r1_item0 = r1;
     b.r2_item0 = r2;
     tile0 += tiles;
     r1 += int64_t(b.ngroups) * b.colblk;
-    r2 += b.colblk;
+    r2 += b.colblk16;
   }
   hdr->n_sets = n_sets;
   hdr->variant = variant;

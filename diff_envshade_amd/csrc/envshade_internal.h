@@ -41,6 +41,7 @@ struct Problem {
   float* gs;
   float* gr;
   float* gE;              // 3 x H x W
+  const int32_t* order;   // P: point of each tile lane (tile*64 + lane), or null = identity
   float* part;            // [ceil(P/64)][3][T] per-tile env-map partials
   double* part2;          // [ngroups][3][T] stage-1 FP64 sums
 };
@@ -64,7 +65,8 @@ struct BatchSet {
   int64_t r1_item0;   // first stage-1 reduce item (ngroups * colblk items per set)
   int64_t r2_item0;   // first stage-2 reduce item (colblk items per set)
   int32_t ngroups;    // ceil(tiles / reduce_rows)
-  int32_t colblk;     // ceil(3 T / 256)
+  int32_t colblk;     // ceil(3 T / 256) (stage 1)
+  int32_t colblk16;   // ceil(3 T / 16) (stage 2)
 };
 
 struct BatchHeader {
@@ -99,7 +101,7 @@ constexpr int kForwardAcc = 6;
 constexpr int kBackwardAcc = 24;
 constexpr int kMaxSuperChunk = 1024;  // texels staged in LDS at once (48 B each, 48 KB)
 constexpr int kTexelBytes = 48;
-constexpr int kReduceRows = 32;
+constexpr int kReduceRows = 16;
 
 hipError_t launch_forward(int variant, bool diff, bool spec, const Problem& pr, const Tiling& tl,
                           hipStream_t stream);
@@ -107,5 +109,18 @@ hipError_t launch_backward(int variant, bool diff, bool spec, const Problem& pr,
                            hipStream_t stream);
 hipError_t launch_envgrad_reduce(const Problem& pr, const Tiling& tl, hipStream_t stream);
 hipError_t launch_fill_zero(float* x, int64_t n, hipStream_t stream);
+
+// ---- normal-coherent point order (point_order.hip) ----
+constexpr int kOrderBlock = 4096;  // points per block of the counting sort
+constexpr int kOrderKeys = 1024;   // 32 x 32 octahedral cells of the normal
+
+// Workspace of the point order, offsets from a 256-byte aligned base; the
+// order itself (int32 x P) is at order_off and stays valid for the shading
+// launches that follow on the same stream.
+struct OrderLayout {
+  size_t order_off, keyrank_off, count_off, total_off, total;
+};
+OrderLayout order_layout(int64_t P);
+hipError_t launch_point_order(const float* normal, int64_t P, void* ws, hipStream_t stream);
 
 }  // namespace des

@@ -175,20 +175,120 @@ __device__ __forceinline__ float vis_weight(const PointState<V, SPEC>& s, const 
   return fmaf(b2, s.c2, fmaf(b1, s.c1, b0 * s.c0));
 }
 
+// Texel groups: a super-chunk of st = ntc * cw texels is cut into groups of 4;
+// wave wv takes group k * ntc + ((wv + k) mod ntc) for k = 0 .. cw/4 - 1, a
+// diagonal across the rows of the lat-long map.  Each wave thus sees every
+// latitude and longitude band, so the waves of a tile get about the same share
+// of the texels that light none of its points (which are skipped), whatever
+// direction the tile's normals face.  Returns the group's first texel within
+// the super-chunk (a multiple of 4; a group never straddles a 32-texel slot).
+__device__ __forceinline__ int group_texel(int k, int wv, int ntc) {
+  return 4 * (k * ntc + ((wv + k) & (ntc - 1)));
+}
+
+// Visibility words of group k of the super-chunk at s0 (sn texels), or zeros
+// past the wave's last group.
+template <int V, bool SPEC>
+__device__ __forceinline__ VisWords load_group_words(const Problem& pr,
+                                                     const PointState<V, SPEC>& st, int s0,
+                                                     int sn, int k, int ng, int wv, int ntc,
+                                                     bool single) {
+  VisWords w{0u, 0u, 0u};
+  if (V == kVis && k < ng) {
+    const int tg = group_texel(k, wv, ntc);
+    if (tg < sn) w = load_slot<V, SPEC>(pr, st, (s0 + tg) >> 5, single);
+  }
+  return w;
+}
+
+// ---- dark texels ----------------------------------------------------------------
+//
+// A pair adds exactly nothing -- to any output or gradient -- unless l.n > 0:
+// the Lambert term is gated on l.n > 0, and the GGX term vanishes with G = 0
+// at max(l.n, 0) = 0.  If the normals of a tile's contributing lanes lie in a
+// cone (unit axis a, half-angle beta), texel l lights none of them when the
+// angle between l and a is at least 90 deg + beta, i.e. when
+//   l.a <= -sin(beta) |l|.
+// The test widens the cone (cos(beta) lowered by 4e-6, sin(beta) raised by
+// 1e-4), far more than the fp32 rounding of either side: a texel it calls dark
+// has l.n < -1e-4 |l| |n| on every lane, which no lane's own fp32 cosine can
+// round up to a positive value, so skipping it changes no result.
+struct Cone {
+  float ax, ay, az;  // axis
+  float sb;          // widened sin(beta); -4: every texel dark, 4: none
+};
+
+__device__ __forceinline__ float uniform(float x) {
+  return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x)));
+}
+
+__device__ __forceinline__ float wave_sum(float x) {
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) x += __shfl_xor(x, d, 64);
+  return x;  // the same in every lane: each step adds the same two values
+}
+
+__device__ __forceinline__ float wave_min(float x) {
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) x = fminf(x, __shfl_xor(x, d, 64));
+  return x;
+}
+
+// Cone of the normals of the lanes that can contribute (live, finite, non-zero).
+template <int V, bool SPEC>
+__device__ __forceinline__ Cone tile_cone(const PointState<V, SPEC>& s) {
+  Cone c{0.f, 0.f, 0.f, 4.f};
+  const float n2 = fmaf(s.nz, s.nz, fmaf(s.ny, s.ny, s.nx * s.nx));
+  const bool inc = s.live != 0.f && n2 > 0.f && n2 < INFINITY;
+  if (!__any(inc)) {
+    c.sb = -4.f;
+    return c;
+  }
+  const float r = inc ? rsq(n2) : 0.f;
+  const float ux = inc ? s.nx * r : 0.f, uy = inc ? s.ny * r : 0.f, uz = inc ? s.nz * r : 0.f;
+  const float sx = uniform(wave_sum(ux)), sy = uniform(wave_sum(uy)), sz = uniform(wave_sum(uz));
+  const float s2 = fmaf(sz, sz, fmaf(sy, sy, sx * sx));
+  if (!(s2 > 1e-6f)) return c;  // normals cancel: no useful cone
+  const float ra = rsq(s2);
+  const float ax = sx * ra, ay = sy * ra, az = sz * ra;
+  const float cmin = uniform(wave_min(inc ? fmaf(uz, az, fmaf(uy, ay, ux * ax)) : 1.f)) - 4e-6f;
+  if (!(cmin > 0.f)) return c;  // wider than a hemisphere
+  c.ax = ax;
+  c.ay = ay;
+  c.az = az;
+  c.sb = __builtin_amdgcn_sqrtf(fmaxf(0.f, fmaf(-cmin, cmin, 1.f))) + 1e-4f;
+  return c;
+}
+
+// Dark bits of the 16 texel groups k0 .. k0+15 of wave wv: bit 4 i + j is
+// texel j of group k0 + i (groups past ng read as dark).
+__device__ __forceinline__ uint64_t dark_mask(const Texel* tab, const Cone& c, int k0, int ng,
+                                              int wv, int ntc, int lane) {
+  const int k = k0 + (lane >> 2);
+  bool dark = true;
+  if (k < ng) {
+    const Texel& g = tab[group_texel(k, wv, ntc) + (lane & 3)];
+    const float len = __builtin_amdgcn_sqrtf(fmaf(g.lz, g.lz, fmaf(g.ly, g.ly, g.lx * g.lx)));
+    dark = fmaf(g.lz, c.az, fmaf(g.ly, c.ay, g.lx * c.ax)) <= -c.sb * len;
+  }
+  return __ballot(dark);
+}
+
 // ---- forward -----------------------------------------------------------------
 
 // One 64-point tile of one problem; called by every wave of the workgroup.
 // Waves other than wave 0 return after the cross-chunk reduction; wave 0
 // writes the outputs.  No barrier follows the last one executed here, so a
 // caller reusing LDS must synchronise first.
-template <int V, bool DIFF, bool SPEC>
+template <int V, bool DIFF, bool SPEC, bool CULL>
 __device__ __forceinline__ void forward_tile(const Problem& pr, const Tiling& tl, int64_t tile) {
   extern __shared__ float4 smem[];
   constexpr bool DOUT = (V == kV2 || V == kVis);
   const int lane = threadIdx.x & 63;
-  const int wv = threadIdx.x >> 6;
-  const int64_t p = tile * 64 + lane;
-  const bool valid = p < pr.P;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
+  const int64_t q = tile * 64 + lane;
+  const bool valid = q < pr.P;
+  const int64_t p = (pr.order != nullptr && valid) ? (int64_t)pr.order[q] : q;
 
   PointState<V, SPEC> st;
   load_point<V, SPEC>(pr, p, valid, st);
@@ -199,6 +299,9 @@ __device__ __forceinline__ void forward_tile(const Problem& pr, const Tiling& tl
     csum = st.c0 + st.c1 + st.c2;
   }
 
+  Cone cone{0.f, 0.f, 0.f, 4.f};
+  if (CULL) cone = tile_cone(st);
+
   float ds0 = 0.f, ds1 = 0.f, ds2 = 0.f;  // sum E dOmega vis max(l.n,0)
   float sp0 = 0.f, sp1 = 0.f, sp2 = 0.f;  // sum E dOmega vis F G D / 4
 
@@ -208,14 +311,23 @@ __device__ __forceinline__ void forward_tile(const Problem& pr, const Tiling& tl
     __syncthreads();
     build_table(pr, s0, sn, tl.st, tab);
     __syncthreads();
-    const int c_begin = wv * tl.cw;
-    const int c_end = min(c_begin + tl.cw, (sn + 31) & ~31);
-    for (int sl = c_begin; sl < c_end; sl += 32) {
-      const VisWords w = load_slot<V, SPEC>(pr, st, (s0 + sl) >> 5, single);
-#pragma unroll 4
-      for (int b = 0; b < 32; ++b) {
-        const Texel& g = tab[sl + b];
-        const float vis = vis_weight<V, SPEC>(st, w, b, single, csum);
+    const int ng = tl.cw >> 2;
+    uint64_t dark = 0ull;
+    // visibility words of the next group are requested one group ahead
+    VisWords wn = load_group_words(pr, st, s0, sn, 0, ng, wv, tl.ntc, single);
+    for (int k = 0; k < ng; ++k) {
+      const int tg = group_texel(k, wv, tl.ntc);
+      if (tg >= sn) break;
+      const VisWords w = wn;
+      wn = load_group_words(pr, st, s0, sn, k + 1, ng, wv, tl.ntc, single);
+      if (CULL && (k & 15) == 0) dark = dark_mask(tab, cone, k, ng, wv, tl.ntc, lane);
+      const uint32_t d4 = CULL ? (uint32_t)(dark >> (4 * (k & 15))) & 0xfu : 0u;
+      if (d4 == 0xfu) continue;  // lights no lane of the tile: adds exactly 0
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        if ((d4 >> i) & 1u) continue;
+        const Texel& g = tab[tg + i];
+        const float vis = vis_weight<V, SPEC>(st, w, (tg & 31) + i, single, csum);
         const Lobe lb = eval_lobe<SPEC>(st, g);
         if (DIFF) {
           const float x = lb.nl * vis;
@@ -280,14 +392,15 @@ __device__ __forceinline__ void forward_tile(const Problem& pr, const Tiling& tl
 //   terms of the NDF and normalize VJPs are accumulated along l + v and along
 //   hb x (n x hb); the G1 parts along n and v are per-point constants times a
 //   scalar sum (Sb), applied once at the end.
-template <int V, bool DIFF, bool SPEC>
+template <int V, bool DIFF, bool SPEC, bool CULL>
 __device__ __forceinline__ void backward_tile(const Problem& pr, const Tiling& tl, int64_t tile) {
   extern __shared__ float4 smem[];
   constexpr bool DOUT = (V == kV2 || V == kVis);
   const int lane = threadIdx.x & 63;
-  const int wv = threadIdx.x >> 6;
-  const int64_t p = tile * 64 + lane;
-  const bool valid = p < pr.P;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
+  const int64_t q0 = tile * 64 + lane;
+  const bool valid = q0 < pr.P;
+  const int64_t p = (pr.order != nullptr && valid) ? (int64_t)pr.order[q0] : q0;
 
   PointState<V, SPEC> st;
   load_point<V, SPEC>(pr, p, valid, st);
@@ -311,6 +424,9 @@ __device__ __forceinline__ void backward_tile(const Problem& pr, const Tiling& t
     }
   }
 
+  Cone cone{0.f, 0.f, 0.f, 4.f};
+  if (CULL) cone = tile_cone(st);
+
   float ds0 = 0.f, ds1 = 0.f, ds2 = 0.f;
   float sp0 = 0.f, sp1 = 0.f, sp2 = 0.f;
   float lnx = 0.f, lny = 0.f, lnz = 0.f;  // sum coef_n l
@@ -332,18 +448,32 @@ __device__ __forceinline__ void backward_tile(const Problem& pr, const Tiling& t
     __syncthreads();
     build_table(pr, s0, sn, tl.st, tab);
     __syncthreads();
-    const int c_begin = wv * tl.cw;
-    const int c_end = min(c_begin + tl.cw, (sn + 31) & ~31);
-    for (int sl = c_begin; sl < c_end; sl += 32) {
-      const VisWords w = load_slot<V, SPEC>(pr, st, (s0 + sl) >> 5, single);
+    const int ng = tl.cw >> 2;
+    uint64_t dark = 0ull;
+    VisWords wn = load_group_words(pr, st, s0, sn, 0, ng, wv, tl.ntc, single);
 #pragma unroll 1
-      for (int b4 = 0; b4 < 32; b4 += 4) {
+    for (int k = 0; k < ng; ++k) {
+      const int tg = group_texel(k, wv, tl.ntc);
+      if (tg >= sn) break;
+      const VisWords w = wn;
+      wn = load_group_words(pr, st, s0, sn, k + 1, ng, wv, tl.ntc, single);
+      if (CULL && (k & 15) == 0) dark = dark_mask(tab, cone, k, ng, wv, tl.ntc, lane);
+      const uint32_t d4 = CULL ? (uint32_t)(dark >> (4 * (k & 15))) & 0xfu : 0u;
+      const int t_out = s0 + tg + row_texel;
+      if (d4 == 0xfu) {
+        // the 4 texels light no lane of the tile: their partials are exactly 0
+        if (ch_out < 3 && t_out < pr.T) part[(int64_t)ch_out * pr.T + t_out] = 0.f;
+        continue;
+      }
+      const int b4 = tg & 31;
+      {
         float q[4][3];
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-          const int b = b4 + i;
-          const Texel& g = tab[sl + b];
-          const float vis = vis_weight<V, SPEC>(st, w, b, single, csum);
+          q[i][0] = q[i][1] = q[i][2] = 0.f;
+          if ((d4 >> i) & 1u) continue;
+          const Texel& g = tab[tg + i];
+          const float vis = vis_weight<V, SPEC>(st, w, b4 + i, single, csum);
           const Lobe lb = eval_lobe<SPEC>(st, g);
           float coef_n = 0.f;
           float x = 0.f, y = 0.f;
@@ -403,9 +533,9 @@ __device__ __forceinline__ void backward_tile(const Problem& pr, const Tiling& t
             hvx = fmaf(q2, lb.hx, hvx);
             hvy = fmaf(q2, lb.hy, hvy);
             hvz = fmaf(q2, lb.hz, hvz);
-            cvx = fmaf(r3, fmaf(lb.hy, lb.cz, -lb.hz * lb.cy), cvx);
-            cvy = fmaf(r3, fmaf(lb.hz, lb.cx, -lb.hx * lb.cz), cvy);
-            cvz = fmaf(r3, fmaf(lb.hx, lb.cy, -lb.hy * lb.cx), cvz);
+            cvx = fmaf(r3, lb.ux, cvx);
+            cvy = fmaf(r3, lb.uy, cvy);
+            cvz = fmaf(r3, lb.uz, cvz);
             const float coef_v = lb.rs * gch;
             lvx = fmaf(coef_v, g.lx, lvx);
             lvy = fmaf(coef_v, g.ly, lvy);
@@ -421,10 +551,9 @@ __device__ __forceinline__ void backward_tile(const Problem& pr, const Tiling& t
         const float r0 = reduce4(q[0][0], q[1][0], q[2][0], q[3][0]);
         const float r1 = reduce4(q[0][1], q[1][1], q[2][1], q[3][1]);
         const float r2 = reduce4(q[0][2], q[1][2], q[2][2], q[3][2]);
-        const int t = s0 + sl + b4 + row_texel;
-        if (ch_out < 3 && t < pr.T) {
+        if (ch_out < 3 && t_out < pr.T) {
           const float val = (ch_out == 0) ? r0 : ((ch_out == 1) ? r1 : r2);
-          part[(int64_t)ch_out * pr.T + t] = val;
+          part[(int64_t)ch_out * pr.T + t_out] = val;
         }
       }
     }
@@ -484,12 +613,12 @@ __device__ __forceinline__ void backward_tile(const Problem& pr, const Tiling& t
 
 template <int V, bool DIFF, bool SPEC>
 __global__ __launch_bounds__(1024) void shade_forward_kernel(Problem pr, Tiling tl) {
-  forward_tile<V, DIFF, SPEC>(pr, tl, blockIdx.x);
+  forward_tile<V, DIFF, SPEC, true>(pr, tl, blockIdx.x);
 }
 
 template <int V, bool DIFF, bool SPEC>
 __global__ __launch_bounds__(512) void shade_backward_kernel(Problem pr, Tiling tl) {
-  backward_tile<V, DIFF, SPEC>(pr, tl, blockIdx.x);
+  backward_tile<V, DIFF, SPEC, true>(pr, tl, blockIdx.x);
 }
 
 // ---- batched launches: persistent workgroups pull tiles of many problems ------
@@ -529,7 +658,7 @@ __global__ __launch_bounds__(512) void batch_forward_kernel(const BatchHeader* _
     if (g >= total) break;
     const int s = __builtin_amdgcn_readfirstlane(
         find_set(n, g, [&](int i) { return sets[i].tile0; }));
-    forward_tile<V, DIFF, SPEC>(sets[s].pr, sets[s].tl_fwd, g - sets[s].tile0);
+    forward_tile<V, DIFF, SPEC, false>(sets[s].pr, sets[s].tl_fwd, g - sets[s].tile0);
   }
 }
 
@@ -545,7 +674,7 @@ __global__ __launch_bounds__(512) void batch_backward_kernel(const BatchHeader* 
     if (g >= total) break;
     const int s = __builtin_amdgcn_readfirstlane(
         find_set(n, g, [&](int i) { return sets[i].tile0; }));
-    backward_tile<V, DIFF, SPEC>(sets[s].pr, sets[s].tl_bwd, g - sets[s].tile0);
+    backward_tile<V, DIFF, SPEC, false>(sets[s].pr, sets[s].tl_bwd, g - sets[s].tile0);
   }
 }
 
@@ -576,19 +705,35 @@ __device__ __forceinline__ void reduce_rows_item(const float* __restrict__ part,
   part2[g * ncol + col] = s;
 }
 
-// stage 2: gE[c][t] = dOmega_t * sum_g part2[g][c*T + t], rounded once to fp32
+// stage 2: gE[c][t] = dOmega_t * sum_g part2[g][c*T + t], rounded once to fp32.
+// A 256-thread block owns 16 columns; thread (column j, lane q) sums groups
+// q, q+16, ... and a fixed-order LDS tree folds the 16 lanes, so the FP64 sum
+// order depends only on ngroups.
 __device__ __forceinline__ void finish_item(const double* __restrict__ part2,
                                             float* __restrict__ gE, int ngroups, int H, int W,
-                                            int cb) {
+                                            int cb16) {
+  __shared__ double s_sum[16][17];
   const int T = H * W;
-  const int col = cb * 256 + threadIdx.x;
-  if (col >= 3 * T) return;
-  const int t = col % T;
-  const int iy = H - 1 - t / W;
+  const int j = threadIdx.x & 15;
+  const int q = threadIdx.x >> 4;
+  const int col = cb16 * 16 + j;
   double s = 0.0;
-  for (int g = 0; g < ngroups; ++g) s += part2[(int64_t)g * 3 * T + col];
-  const double dw = solid_angle_scale(H, W) * sinpi(((double)iy + 0.5) / (double)H);
-  gE[col] = (float)(s * dw);
+  if (col < 3 * T)
+    for (int g = q; g < ngroups; g += 16) s += part2[(int64_t)g * 3 * T + col];
+  s_sum[q][j] = s;
+  __syncthreads();
+#pragma unroll
+  for (int step = 8; step > 0; step >>= 1) {
+    if (q < step) s_sum[q][j] += s_sum[q + step][j];
+    __syncthreads();
+  }
+  if (q == 0 && col < 3 * T) {
+    const int t = col % T;
+    const int iy = H - 1 - t / W;
+    const double dw = solid_angle_scale(H, W) * sinpi(((double)iy + 0.5) / (double)H);
+    gE[col] = (float)(s_sum[0][j] * dw);
+  }
+  __syncthreads();
 }
 
 __global__ __launch_bounds__(256) void reduce_partials_kernel(const float* __restrict__ part,
@@ -625,7 +770,8 @@ __global__ __launch_bounds__(256) void batch_finish_envgrad_kernel(
   const int64_t total = hdr->total_r2;
   const int n = hdr->n_sets;
   for (int64_t item = blockIdx.x; item < total; item += gridDim.x) {
-    const int s = find_set(n, item, [&](int i) { return sets[i].r2_item0; });
+    const int s = __builtin_amdgcn_readfirstlane(
+        find_set(n, item, [&](int i) { return sets[i].r2_item0; }));
     const int cb = (int)(item - sets[s].r2_item0);
     const Problem& pr = sets[s].pr;
     finish_item(pr.part2, pr.gE, sets[s].ngroups, pr.H, pr.W, cb);
@@ -720,8 +866,8 @@ hipError_t launch_envgrad_reduce(const Problem& pr, const Tiling& tl, hipStream_
                      pr.part2, nrows, ncol, tl.reduce_rows);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(finish_envgrad_kernel, dim3(cb), dim3(256), 0, stream, pr.part2, pr.gE,
-                     ngroups, pr.H, pr.W);
+  hipLaunchKernelGGL(finish_envgrad_kernel, dim3((unsigned)((ncol + 15) / 16)), dim3(256), 0,
+                     stream, pr.part2, pr.gE, ngroups, pr.H, pr.W);
   return hipGetLastError();
 }
 

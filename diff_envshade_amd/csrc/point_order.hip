@@ -1,0 +1,272 @@
+// point_order.hip -- normal-coherent point order for the shading kernels.
+//
+// A tile of 64 points whose normals point the same way leaves whole texels
+// with no lit lane: every texel in the opposite hemisphere of all 64 normals
+// has l.n <= 0 on every lane and contributes exactly nothing, so the shading
+// kernels skip it.  With random point order a tile's normals cover the sphere
+// and no texel is ever dead; sorted by normal direction about 45% of the
+// (tile, texel) pairs of a 16x32 map are.
+//
+// Key: the 10-bit Morton code of the normal's cell on a 32 x 32 octahedral
+// map, so consecutive keys are neighbouring directions.  The order is a
+// stable counting sort by key -- points with equal keys keep their index
+// order -- computed without atomics on positions, so it depends only on the
+// normals, and the per-tile env-map-gradient partials (whose summation order
+// follows tile membership) are bitwise reproducible.  Per-point outputs do not
+// depend on the order at all: each point's texel sums run in the same order
+// whatever tile it lands in.
+//
+// Blocks of 4096 points (1024 threads, 4 rounds of one point per thread):
+//   1. key of each point and its rank among the equal-key points of its block
+//      that precede it (within a wave: a match mask built from one ballot per
+//      key bit; across waves and rounds: per-key counts in LDS), plus the
+//      block's per-key counts, stored key-major: count[key][block];
+//   2. (more than 64 blocks only) per key, an exclusive scan of count[key][.]
+//      over blocks in place, and the key totals;
+//   3. key bases (an exclusive scan over keys of the totals, in LDS) and the
+//      scatter: each point goes to base[key] + prefix[key][block] + rank.  Up
+//      to 64 blocks, each block sums the count rows itself (step 2 fused).
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "envshade_internal.h"
+
+namespace des {
+
+namespace {
+
+constexpr int kThreads = 1024;
+constexpr int kRounds = kOrderBlock / kThreads;  // 4
+constexpr int kWaves = kThreads / 64;             // 16
+constexpr int kKeys = kOrderKeys;                 // 1024
+constexpr int kKeyBits = 10;
+constexpr int kFusedBlocks = 64;                  // fused scan up to 64 blocks (262144 points)
+
+// bits b0..b4 of x -> bit positions 0, 2, 4, 6, 8
+__device__ __forceinline__ uint32_t spread5(uint32_t x) {
+  x &= 31u;
+  x = (x | (x << 4)) & 0x0f0fu;
+  x = (x | (x << 2)) & 0x3333u;
+  x = (x | (x << 1)) & 0x5555u;
+  return x;
+}
+
+// 10-bit Morton code of the 32 x 32 octahedral cell of direction (x, y, z).
+// Zero or non-finite normals get key 0.
+__device__ __forceinline__ uint32_t normal_key(float x, float y, float z) {
+  const float s = fabsf(x) + fabsf(y) + fabsf(z);
+  if (!(s > 0.f) || !(s < INFINITY)) return 0u;
+  float u = x / s, v = y / s;
+  if (z < 0.f) {
+    const float fu = (1.f - fabsf(v)) * (u < 0.f ? -1.f : 1.f);
+    const float fv = (1.f - fabsf(u)) * (v < 0.f ? -1.f : 1.f);
+    u = fu;
+    v = fv;
+  }
+  const int iu = min(max((int)((u + 1.f) * 16.f), 0), 31);
+  const int iv = min(max((int)((v + 1.f) * 16.f), 0), 31);
+  return spread5((uint32_t)iu) | (spread5((uint32_t)iv) << 1);
+}
+
+__device__ __forceinline__ uint64_t lanemask_lt(int lane) {
+  return lane == 0 ? 0ull : (~0ull >> (64 - lane));
+}
+
+// 1. keys, in-block ranks, per-block key counts
+__global__ __launch_bounds__(kThreads) void order_rank_kernel(const float* __restrict__ normal,
+                                                              int64_t P, uint32_t* __restrict__ keyrank,
+                                                              uint32_t* __restrict__ count, int nblk,
+                                                              int nblk_pad) {
+  // wcount[w][k] = (round + 1) << 12 | n: first the number n of key-k points of
+  // wave w in that round, then (after the column scan) the number of key-k
+  // points of this block that precede wave w's
+  __shared__ uint16_t wcount[kWaves][kKeys];  // 32 KB
+  __shared__ uint16_t run[kKeys];             // key-k points of this block in earlier rounds
+  static_assert(kKeys == kThreads, "thread k owns key k");
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wv = tid >> 6;
+  const int blk = blockIdx.x;
+  // all four rounds' normals are requested before any is used: one memory
+  // latency per block instead of one per round
+  float nrm[kRounds][3];
+#pragma unroll
+  for (int r = 0; r < kRounds; ++r) {
+    const int64_t p = (int64_t)blk * kOrderBlock + r * kThreads + tid;
+    nrm[r][0] = nrm[r][1] = nrm[r][2] = 0.f;
+    if (p < P) {
+      nrm[r][0] = normal[3 * p];
+      nrm[r][1] = normal[3 * p + 1];
+      nrm[r][2] = normal[3 * p + 2];
+    }
+  }
+  for (int i = tid; i < kWaves * kKeys; i += kThreads) (&wcount[0][0])[i] = 0;
+  run[tid] = 0;
+  __syncthreads();
+  // ranks stay in registers until the end: a barrier waits for outstanding
+  // global stores, so none are issued between barriers
+  uint32_t kr[kRounds];
+#pragma unroll
+  for (int r = 0; r < kRounds; ++r) {
+    const int64_t p = (int64_t)blk * kOrderBlock + r * kThreads + tid;
+    const bool valid = p < P;
+    const uint32_t key = valid ? normal_key(nrm[r][0], nrm[r][1], nrm[r][2]) : 0u;
+    // lanes of this wave holding the same key: intersect, over the key bits,
+    // the ballot of lanes that agree with this lane on the bit
+    uint64_t eq = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < kKeyBits; ++b) {
+      const bool bit = (key >> b) & 1u;
+      const uint64_t m = __ballot(bit);
+      eq &= bit ? m : ~m;
+    }
+    const uint32_t rank_w = (uint32_t)__popcll(eq & lanemask_lt(lane));
+    const uint16_t tag = (uint16_t)((r + 1) << 12);
+    if (valid && rank_w == 0u) wcount[wv][key] = (uint16_t)(tag | (uint16_t)__popcll(eq));
+    __syncthreads();
+    // column scan of key tid over the waves: counts of this round -> prefixes
+    {
+      uint32_t acc = run[tid];
+#pragma unroll
+      for (int w = 0; w < kWaves; ++w) {
+        const uint16_t c = wcount[w][tid];
+        if ((c & 0xf000) == tag) {
+          wcount[w][tid] = (uint16_t)(tag | acc);
+          acc += c & 0x0fffu;
+        }
+      }
+      run[tid] = (uint16_t)acc;
+    }
+    __syncthreads();
+    kr[r] = valid ? ((key << 16) | ((uint32_t)(wcount[wv][key] & 0x0fff) + rank_w)) : 0u;
+  }
+#pragma unroll
+  for (int r = 0; r < kRounds; ++r) {
+    const int64_t p = (int64_t)blk * kOrderBlock + r * kThreads + tid;
+    if (p < P) keyrank[p] = kr[r];
+  }
+  uint32_t* row = count + (int64_t)tid * nblk_pad;
+  row[blk] = run[tid];
+  if (blk == nblk - 1)
+    for (int j = nblk; j < nblk_pad; ++j) row[j] = 0u;  // row padding read by the fused scan
+}
+
+// inclusive scan over the 64 lanes of a wave
+__device__ __forceinline__ uint32_t wave_scan(uint32_t x, int lane) {
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t y = __shfl_up(x, d, 64);
+    if (lane >= d) x += y;
+  }
+  return x;
+}
+
+// 2. per key: exclusive scan of count[key][0..nblk) in place, total[key]
+__global__ __launch_bounds__(256) void order_scan_kernel(uint32_t* __restrict__ count,
+                                                         uint32_t* __restrict__ total, int nblk,
+                                                         int nblk_pad) {
+  const int lane = threadIdx.x & 63;
+  const int key = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (key >= kKeys) return;
+  uint32_t* row = count + (int64_t)key * nblk_pad;
+  uint32_t carry = 0u;
+  for (int b0 = 0; b0 < nblk; b0 += 64) {
+    const int b = b0 + lane;
+    const uint32_t c = b < nblk ? row[b] : 0u;
+    const uint32_t inc = wave_scan(c, lane);
+    if (b < nblk) row[b] = carry + inc - c;
+    carry += __shfl(inc, 63, 64);
+  }
+  if (lane == 0) total[key] = carry;
+}
+
+// 3. key bases, then scatter.  fused: count holds the raw per-block counts
+// (rows padded with zeros to nblk_pad, a multiple of 4); otherwise the
+// exclusive prefixes of step 2 and total[] the key totals.
+__global__ __launch_bounds__(kThreads) void order_scatter_kernel(
+    const uint32_t* __restrict__ keyrank, const uint32_t* __restrict__ count,
+    const uint32_t* __restrict__ total, int64_t P, int nblk_pad, int fused,
+    int32_t* __restrict__ order) {
+  __shared__ uint32_t off[kKeys];
+  __shared__ uint32_t wsum[kWaves];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wv = tid >> 6;
+  const int blk = blockIdx.x;
+  uint32_t kr[kRounds];
+#pragma unroll
+  for (int r = 0; r < kRounds; ++r) {
+    const int64_t p = (int64_t)blk * kOrderBlock + r * kThreads + tid;
+    kr[r] = p < P ? keyrank[p] : 0u;
+  }
+  uint32_t tot = 0u, pre = 0u;
+  if (fused) {
+    const uint4* row = reinterpret_cast<const uint4*>(count + (int64_t)tid * nblk_pad);
+    for (int j4 = 0; j4 < (nblk_pad >> 2); ++j4) {
+      const uint4 c = row[j4];
+      const int j = 4 * j4;
+      tot += (c.x + c.y) + (c.z + c.w);
+      pre += (j < blk ? c.x : 0u) + (j + 1 < blk ? c.y : 0u) + (j + 2 < blk ? c.z : 0u) +
+             (j + 3 < blk ? c.w : 0u);
+    }
+  } else {
+    tot = total[tid];
+    pre = count[(int64_t)tid * nblk_pad + blk];
+  }
+  const uint32_t inc = wave_scan(tot, lane);
+  if (lane == 63) wsum[wv] = inc;
+  __syncthreads();
+  uint32_t o = 0u;
+  for (int w = 0; w < wv; ++w) o += wsum[w];
+  off[tid] = o + inc - tot + pre;
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < kRounds; ++r) {
+    const int64_t p = (int64_t)blk * kOrderBlock + r * kThreads + tid;
+    if (p < P) order[off[kr[r] >> 16] + (kr[r] & 0xffffu)] = (int32_t)p;
+  }
+}
+
+}  // namespace
+
+OrderLayout order_layout(int64_t P) {
+  OrderLayout L{0, 0, 0, 0, 0};
+  if (P <= 0) return L;
+  const int64_t nblk = (P + kOrderBlock - 1) / kOrderBlock;
+  const int64_t nblk_pad = (nblk + 3) & ~int64_t(3);
+  auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
+  L.order_off = 0;
+  L.keyrank_off = al(size_t(P) * 4);
+  L.count_off = L.keyrank_off + al(size_t(P) * 4);
+  L.total_off = L.count_off + al(size_t(kKeys) * size_t(nblk_pad) * 4);
+  L.total = L.total_off + al(size_t(kKeys) * 4);
+  return L;
+}
+
+hipError_t launch_point_order(const float* normal, int64_t P, void* ws, hipStream_t stream) {
+  if (P <= 0) return hipSuccess;
+  const OrderLayout L = order_layout(P);
+  char* base = static_cast<char*>(ws);
+  int32_t* order = reinterpret_cast<int32_t*>(base + L.order_off);
+  uint32_t* keyrank = reinterpret_cast<uint32_t*>(base + L.keyrank_off);
+  uint32_t* count = reinterpret_cast<uint32_t*>(base + L.count_off);
+  uint32_t* total = reinterpret_cast<uint32_t*>(base + L.total_off);
+  const int nblk = (int)((P + kOrderBlock - 1) / kOrderBlock);
+  const int nblk_pad = (nblk + 3) & ~3;
+  const bool fused = nblk <= kFusedBlocks;
+  hipLaunchKernelGGL(order_rank_kernel, dim3(nblk), dim3(kThreads), 0, stream, normal, P, keyrank,
+                     count, nblk, nblk_pad);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  if (!fused) {
+    hipLaunchKernelGGL(order_scan_kernel, dim3((kKeys + 3) / 4), dim3(256), 0, stream, count,
+                       total, nblk, nblk_pad);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(order_scatter_kernel, dim3(nblk), dim3(kThreads), 0, stream, keyrank, count,
+                     total, P, nblk_pad, fused ? 1 : 0, order);
+  return hipGetLastError();
+}
+
+}  // namespace des

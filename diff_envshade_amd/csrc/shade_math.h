@@ -88,7 +88,7 @@ struct Lobe {
   float cs;    // l.n
   float nl;    // max(cs, 0)
   float hx, hy, hz;  // hb = l + v (formed in FP64)
-  float cx, cy, cz;  // n x hb (formed in FP64)
+  float ux, uy, uz;  // hb x (n x hb) = n |hb|^2 - hb (n.hb) (formed in FP64)
   float rs;    // 1/|l+v|
   float ch;    // l.h
   float nh;    // n.h
@@ -102,15 +102,18 @@ struct Lobe {
 };
 
 // S provides nx..nz, vx..vz (fp32), nxd..nzd, vxd..vzd (the same in FP64),
-// cn = 1 - |n|^2 (from FP64), k, omk, ib, a2, am1 (per-point constants).
+// k, omk, ib, a2, am1 (per-point constants).
 //
 // Geometry of the half vector in FP64 (v_fma_f64 runs at the fp32 vector rate
-// on gfx950): hb = l + v, |hb|^2, n.hb and |n x hb|^2 are formed exactly
-// enough that the NDF denominator
+// on gfx950): hb = l + v, |hb|^2 and n.hb are formed exactly enough that the
+// NDF denominator
 //   tD = (n.h)^2 (alpha^2 - 1) + 1 = alpha^2 (n.h)^2 + (1 - (n.h)^2),
-//   1 - (n.h)^2 = (|n x hb|^2 + (1 - |n|^2) |hb|^2) / |hb|^2   (Lagrange identity)
+//   1 - (n.h)^2 = (|hb|^2 - (n.hb)^2) / |hb|^2   (numerator in FP64)
 // keeps full fp32 relative accuracy at the peak of sharp lobes, where the
-// reference's fp32 form cancels (1 - (n.h)^2 ~ alpha^2 ~ 1e-4).
+// reference's fp32 form cancels (1 - (n.h)^2 ~ alpha^2 ~ 1e-4).  The identity
+// holds for any |n|, as the reference's form does.  The backward's
+// n - (n.h) h = (hb x (n x hb)) / |hb|^2 is formed the same way, as
+// n |hb|^2 - hb (n.hb) in FP64 (size ~ sin(theta_h), no 1/sin cancellation).
 template <bool SPEC, class S>
 __device__ __forceinline__ Lobe eval_lobe(const S& s, const Texel& g) {
   Lobe o;
@@ -121,13 +124,10 @@ __device__ __forceinline__ Lobe eval_lobe(const S& s, const Texel& g) {
     const double hx = g.dx + s.vxd, hy = g.dy + s.vyd, hz = g.dz + s.vzd;
     const double hb2 = fma(hz, hz, fma(hy, hy, hx * hx));
     const double nhb = fma(s.nzd, hz, fma(s.nyd, hy, s.nxd * hx));
-    const double cx = fma(s.nyd, hz, -s.nzd * hy);
-    const double cy = fma(s.nzd, hx, -s.nxd * hz);
-    const double cz = fma(s.nxd, hy, -s.nyd * hx);
-    const float c2 = (float)fma(cz, cz, fma(cy, cy, cx * cx));
-    o.cx = (float)cx;
-    o.cy = (float)cy;
-    o.cz = (float)cz;
+    const float sn2 = (float)fma(-nhb, nhb, hb2);  // |hb|^2 (1 - (n.h)^2)
+    o.ux = (float)fma(s.nxd, hb2, -hx * nhb);
+    o.uy = (float)fma(s.nyd, hb2, -hy * nhb);
+    o.uz = (float)fma(s.nzd, hb2, -hz * nhb);
     o.hx = (float)hx;
     o.hy = (float)hy;
     o.hz = (float)hz;
@@ -143,7 +143,7 @@ __device__ __forceinline__ Lobe eval_lobe(const S& s, const Texel& g) {
     o.F = fmaf(o.om4 * om, 1.f - kF0, kF0);
     o.ia = rcp(fmaf(o.nl, s.omk, s.k));
     o.G = o.nl * o.ia * s.ib;
-    o.tD = fmaf(o.nh * o.nh, s.a2, fmaf(s.cn, hb2f, c2) * rs2);
+    o.tD = fmaf(o.nh * o.nh, s.a2, sn2 * rs2);
     o.it2 = rcp(fmaxf(kPi * o.tD * o.tD, kThres));
     o.D = s.a2 * o.it2;
     const float tmp = o.F * o.G * o.D * 0.25f;
@@ -152,7 +152,7 @@ __device__ __forceinline__ Lobe eval_lobe(const S& s, const Texel& g) {
     o.tmp = o.spec_on ? tmp : 0.f;
   } else {
     o.hx = o.hy = o.hz = 0.f;
-    o.cx = o.cy = o.cz = 0.f;
+    o.ux = o.uy = o.uz = 0.f;
     o.rs = o.ch = o.nh = o.om4 = o.F = o.G = o.D = o.ia = o.it2 = o.tD = 0.f;
     o.spec_on = false;
     o.tmp = 0.f;
@@ -164,7 +164,7 @@ __device__ __forceinline__ Lobe eval_lobe(const S& s, const Texel& g) {
 struct PointGeom {
   float nx, ny, nz, vx, vy, vz;
   double nxd, nyd, nzd, vxd, vyd, vzd;
-  float nv, cn, k, omk, ib, a2, am1;
+  float nv, k, omk, ib, a2, am1;
 };
 
 __device__ __forceinline__ PointGeom point_geom(float nx, float ny, float nz, float vx, float vy,
@@ -175,7 +175,6 @@ __device__ __forceinline__ PointGeom point_geom(float nx, float ny, float nz, fl
   c.nxd = nx; c.nyd = ny; c.nzd = nz;
   c.vxd = vx; c.vyd = vy; c.vzd = vz;
   c.nv = nx * vx + ny * vy + nz * vz;
-  c.cn = (float)(1.0 - fma(c.nzd, c.nzd, fma(c.nyd, c.nyd, c.nxd * c.nxd)));
   // bs:2126-2129: alpha2 = r^4, k = (r+1)^2 / 8
   const float r2 = r * r;
   c.a2 = r2 * r2;

@@ -48,6 +48,13 @@ class Request:
         return None if self.done_s is None else self.done_s - self.arrival_s
 
 
+REDUCE_ROWS = 16  # partial rows per stage-1 group of the env-gradient reduce
+
+
+def _align256(x: int) -> int:
+    return (x + 255) // 256 * 256
+
+
 def request_bytes(points: int, texels: int, backward: bool, variant: str = "vis") -> int:
     """Device bytes a request holds while in flight: its outputs, plus the
     backward gradients and env-map-gradient workspace."""
@@ -56,7 +63,10 @@ def request_bytes(points: int, texels: int, backward: bool, variant: str = "vis"
         return out
     tiles = (points + 63) // 64
     grads = points * (3 + 3 + 3 + 3 + 1) * 4 + 3 * texels * 4
-    ws = tiles * 3 * texels * 4 + ((tiles + 31) // 32) * 3 * texels * 8 + 512
+    # per-tile fp32 partials, then fp64 sums of REDUCE_ROWS tiles each, both
+    # 256-byte aligned (the library's des_backward_workspace_size)
+    ws = _align256(tiles * 3 * texels * 4) + \
+        _align256(((tiles + REDUCE_ROWS - 1) // REDUCE_ROWS) * 3 * texels * 8)
     return out + grads + ws
 
 

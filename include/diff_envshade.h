@@ -22,8 +22,17 @@
  *     caller may pass uninitialised buffers;
  *   - the backward entry points need a caller-provided workspace of at least
  *     des_backward_workspace_size(P, H, W) bytes (256-byte aligned) for the
- *     per-tile env-map-gradient partials; grad_envmap is summed from them in a
- *     fixed order in FP64, so it is bitwise reproducible run to run;
+ *     per-tile env-map-gradient partials (and the point order, below);
+ *     grad_envmap is summed from them in a fixed order in FP64, so it is
+ *     bitwise reproducible run to run;
+ *   - point order: for large problems the entry points first sort the points
+ *     by normal direction (a stable counting sort, deterministic), so that a
+ *     64-point tile shares a hemisphere and texels lit by none of its points
+ *     are skipped.  Per-point outputs do not depend on the order; grad_envmap
+ *     is summed per tile, so it changes with the order at fp32 rounding level
+ *     and is still bitwise reproducible.  The forward entry points take an
+ *     optional workspace of des_forward_workspace_size(P, H, W) bytes for the
+ *     order (NULL = shade in input order); the backward workspace includes it;
  *   - return 0 (DES_OK) or a negative DES_ERR_* code; des_last_error() gives
  *     the message of the calling thread's last failure.
  *   - calls on distinct streams with distinct buffers are independent; calls
@@ -64,13 +73,22 @@ DES_API const char* des_last_error(void);
  * H x W env map (0 when P == 0). */
 DES_API size_t des_backward_workspace_size(int64_t P, int32_t H, int32_t W);
 
+/* Workspace bytes the *_forward* entry points use for the point order of P
+ * points and an H x W env map (0 when they would not reorder). */
+DES_API size_t des_forward_workspace_size(int64_t P, int32_t H, int32_t W);
+
+/* Point-order policy (no counterpart in the reference), process-wide:
+ * 0 = automatic (reorder when P x H x W is large), 1 = always, -1 = never.
+ * The workspace sizes above follow the policy in force when they are asked. */
+DES_API int des_set_point_order(int32_t mode);
+
 /* bindings.cpp:6 -> bruteforce_diffuse_shader_forward (bruteforce_shader.cu:2687-2731).
  * Lambert only: shading = sum_t [l.n>0] E a (l.n/pi) dOmega. */
 DES_API int des_bruteforce_diffuse_shader_forward(
     int64_t P, int32_t C, int32_t H, int32_t W,
     const float* normal, const float* albedo, const float* envmap,
     const float* light2obj_rotmat,
-    float* shading, void* stream);
+    float* shading, void* workspace, size_t workspace_bytes, void* stream);
 
 /* bindings.cpp:7 -> bruteforce_diffuse_shader_backward (bruteforce_shader.cu:2733-2792).
  * Outputs [grad_normal, grad_albedo, grad_envmap]. */
@@ -88,7 +106,7 @@ DES_API int des_bruteforce_specular_shader_forward(
     const float* normal, const float* view_dir, const float* albedo,
     const float* specular_albedo, const float* roughness, const float* envmap,
     const float* light2obj_rotmat, int enable_diffuse, int enable_specular,
-    float* shading, void* stream);
+    float* shading, void* workspace, size_t workspace_bytes, void* stream);
 
 /* bindings.cpp:10 -> bruteforce_specular_shader_backward (bruteforce_shader.cu:2853-2949).
  * Outputs [grad_normal, grad_view_dir, grad_albedo, grad_specular_albedo,
@@ -111,7 +129,7 @@ DES_API int des_bruteforce_specular_shader_clamp_forward(
     const float* normal, const float* view_dir, const float* albedo,
     const float* specular_albedo, const float* roughness, const float* envmap,
     const float* light2obj_rotmat, int enable_diffuse, int enable_specular,
-    float* shading, void* stream);
+    float* shading, void* workspace, size_t workspace_bytes, void* stream);
 
 /* bindings.cpp:13 -> bruteforce_specular_shader_clamp_backward (bruteforce_shader.cu:3010-3106). */
 DES_API int des_bruteforce_specular_shader_clamp_backward(
@@ -131,7 +149,8 @@ DES_API int des_bruteforce_specular_shader_forward2(
     const float* normal, const float* view_dir, const float* albedo,
     const float* specular_albedo, const float* roughness, const float* envmap,
     const float* light2obj_rotmat, int enable_specular,
-    float* shading, float* diffuse_shading, void* stream);
+    float* shading, float* diffuse_shading, void* workspace, size_t workspace_bytes,
+    void* stream);
 
 /* bindings.cpp:16 -> bruteforce_specular_shader_backward2 (bruteforce_shader.cu:3167-3259). */
 DES_API int des_bruteforce_specular_shader_backward2(
@@ -156,7 +175,8 @@ DES_API int des_bruteforce_specularvisibility_shader_forward2(
     const int32_t* face_vertex_list, const int64_t* nearest_triangle_id,
     const float* barycentric_coord, const int32_t* visibility,
     int enable_specular,
-    float* shading, float* diffuse_shading, void* stream);
+    float* shading, float* diffuse_shading, void* workspace, size_t workspace_bytes,
+    void* stream);
 
 /* bindings.cpp:19 -> bruteforce_specularvisibility_shader_backward2
  * (bruteforce_shader.cu:3338-3453). */

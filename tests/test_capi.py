@@ -36,7 +36,7 @@ def test_library_exports_every_declared_symbol():
     L = _lib.load()
     missing = [n for n in declared_symbols() if not hasattr(L, n)]
     assert not missing, missing
-    assert L.des_abi_version() == 1
+    assert L.des_abi_version() == 2
 
 
 def test_library_links_no_torch():
@@ -56,10 +56,60 @@ def test_workspace_size():
     assert _lib.workspace_size(1, 16, 32) > 0
 
 
+@pytest.mark.parametrize("P,H,W", [(1, 16, 32), (70000, 16, 32), (300000, 32, 64), (4096, 8, 8)])
+def test_serving_charges_the_batch_workspace(P, H, W):
+    # the server shades through the batch API: a request is charged its share
+    # of the batch workspace (all of it minus the descriptor table)
+    from diff_envshade_amd.batch import DesSet, _bind
+    from diff_envshade_amd.serving import request_bytes
+    L = _bind(_lib.load())
+    s = DesSet()
+    s.P, s.H, s.W = P, H, W
+    ws, st = ctypes.c_size_t(0), ctypes.c_size_t(0)
+    assert L.des_batch_plan(4, ctypes.byref(s), 1, ctypes.byref(ws), ctypes.byref(st)) == 0
+    grads = P * 13 * 4 + 3 * H * W * 4
+    outs = P * 3 * 4 * 2
+    assert request_bytes(P, H * W, True) - grads - outs == ws.value - st.value
+
+
+def test_forward_workspace_follows_point_order_policy():
+    try:
+        _lib.set_point_order(-1)
+        assert _lib.forward_workspace_size(70000, 16, 32) == 0
+        never = _lib.workspace_size(70000, 16, 32)
+        _lib.set_point_order(1)
+        fw = _lib.forward_workspace_size(100, 4, 8)
+        assert fw >= 100 * 4 and fw % 256 == 0
+        assert _lib.workspace_size(70000, 16, 32) == never + _lib.forward_workspace_size(70000, 16, 32)
+        _lib.set_point_order(0)
+        # automatic: the headline problem (35.8 M pairs) is reordered, a tiny one is not
+        assert _lib.forward_workspace_size(70000, 16, 32) > 0
+        assert _lib.forward_workspace_size(100, 4, 8) == 0
+        with pytest.raises(RuntimeError):
+            _lib.set_point_order(2)
+    finally:
+        _lib.set_point_order(0)
+
+
+def test_forward_rejects_small_order_workspace():
+    L = _lib.load()
+    try:
+        _lib.set_point_order(1)
+        need = _lib.forward_workspace_size(10, 4, 8)
+        rc = L.des_bruteforce_specularvisibility_shader_forward2(
+            10, 3, 4, 8, 10, 10, 1, *([DUMMY] * 11), 1, DUMMY, DUMMY, DUMMY, need - 1, None)
+        assert rc == -3
+        rc = L.des_bruteforce_specularvisibility_shader_forward2(
+            10, 3, 4, 8, 10, 10, 1, *([DUMMY] * 11), 1, DUMMY, DUMMY, DUMMY + 4, need, None)
+        assert rc == -1
+    finally:
+        _lib.set_point_order(0)
+
+
 def _vis_fwd(P=10, C=3, H=4, W=8, NF=10, NV=10, NS=1, ptr=DUMMY):
     L = _lib.load()
     return L.des_bruteforce_specularvisibility_shader_forward2(
-        P, C, H, W, NF, NV, NS, *([ptr] * 11), 1, ptr, ptr, None)
+        P, C, H, W, NF, NV, NS, *([ptr] * 11), 1, ptr, ptr, None, 0, None)
 
 
 def test_channels_must_be_three():

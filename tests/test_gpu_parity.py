@@ -11,6 +11,17 @@ from _helpers import TOL, compare, gpu_run, oracle_run, problem, relerr
 
 pytestmark = pytest.mark.gpu
 
+
+@pytest.fixture(autouse=True, params=[0, 1], ids=["order-auto", "order-sorted"])
+def point_order(request):
+    """Every parity case runs twice: with the automatic point-order policy
+    (these small problems are shaded in input order) and with the points
+    always sorted by normal direction (dark texels skipped per tile)."""
+    from diff_envshade_amd import _lib
+    _lib.set_point_order(request.param)
+    yield request.param
+    _lib.set_point_order(0)
+
 VARIANT_FLAGS = [
     ("diffuse", True, False),
     ("specular", True, True), ("specular", True, False), ("specular", False, True),
