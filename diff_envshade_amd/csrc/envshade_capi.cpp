@@ -67,9 +67,13 @@ des::Tiling tiling_for(int ntc, int T, bool backward) {
   tl.ntc = ntc;
   tl.cw = slots_per_wave * 32;
   tl.st = ntc * tl.cw;
+  // texel table, then the tile's visibility words (3 per point per slot);
+  // the cross-wave reduction at the end reuses the space from offset 0
   const size_t table = size_t(des::kTexelBytes) * tl.st;
-  tl.lds_fwd = std::max(table, size_t(ntc) * des::kForwardAcc * 64 * sizeof(float));
-  tl.lds_bwd = std::max(table, size_t(ntc) * des::kBackwardAcc * 64 * sizeof(float));
+  const size_t staged = table + size_t(tl.st / 32) * 3 * 64 * sizeof(uint32_t);
+  tl.vis_off = int(table);
+  tl.lds_fwd = std::max(staged, size_t(ntc) * des::kForwardAcc * 64 * sizeof(float));
+  tl.lds_bwd = std::max(staged, size_t(ntc) * des::kBackwardAcc * 64 * sizeof(float));
   tl.reduce_rows = des::kReduceRows;
   (void)backward;
   return tl;

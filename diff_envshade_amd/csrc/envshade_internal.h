@@ -51,6 +51,7 @@ struct Tiling {
   int ntc;          // waves (texel chunks) per workgroup
   int cw;           // texels per wave per super-chunk (multiple of 32)
   int st;           // texels per super-chunk = ntc * cw
+  int vis_off;      // byte offset of the staged visibility words in dynamic LDS
   size_t lds_fwd;   // dynamic LDS bytes, forward
   size_t lds_bwd;   // dynamic LDS bytes, backward
   int reduce_rows;  // partial rows summed per stage-1 group
@@ -111,7 +112,7 @@ hipError_t launch_envgrad_reduce(const Problem& pr, const Tiling& tl, hipStream_
 hipError_t launch_fill_zero(float* x, int64_t n, hipStream_t stream);
 
 // ---- normal-coherent point order (point_order.hip) ----
-constexpr int kOrderBlock = 4096;  // points per block of the counting sort
+constexpr int kOrderBlock = 1024;  // points per block of the counting sort
 constexpr int kOrderKeys = 1024;   // 32 x 32 octahedral cells of the normal
 
 // Workspace of the point order, offsets from a 256-byte aligned base; the

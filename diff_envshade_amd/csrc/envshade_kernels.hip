@@ -146,21 +146,6 @@ struct VisWords {
   uint32_t w0, w1, w2;
 };
 
-template <int V, bool SPEC>
-__device__ __forceinline__ VisWords load_slot(const Problem& pr, const PointState<V, SPEC>& s,
-                                              int slot, bool single) {
-  VisWords w{0u, 0u, 0u};
-  if (V == kVis) {
-    const uint32_t* base = pr.vis + (int64_t)slot * pr.N_vertex;
-    w.w0 = base[s.vert0];
-    if (!single) {
-      w.w1 = base[s.vert1];
-      w.w2 = base[s.vert2];
-    }
-  }
-  return w;
-}
-
 // vis weight for bit b: sum_j c_j * bit_j (bs:2186-2190).  `single` is the
 // wave-uniform case where all three vertices of every lane coincide (one word
 // per slot, weight c0+c1+c2).
@@ -186,17 +171,36 @@ __device__ __forceinline__ int group_texel(int k, int wv, int ntc) {
   return 4 * (k * ntc + ((wv + k) & (ntc - 1)));
 }
 
-// Visibility words of group k of the super-chunk at s0 (sn texels), or zeros
-// past the wave's last group.
+// The tile's visibility words for the slots of the super-chunk at s0 (sn
+// texels) are staged in LDS once per super-chunk -- word of vertex j of lane l
+// for local slot sl at vis[(sl * 3 + j) * 64 + l] -- so that the texel loop
+// reads them from LDS instead of waiting on global memory per group.
 template <int V, bool SPEC>
-__device__ __forceinline__ VisWords load_group_words(const Problem& pr,
-                                                     const PointState<V, SPEC>& st, int s0,
-                                                     int sn, int k, int ng, int wv, int ntc,
-                                                     bool single) {
+__device__ __forceinline__ void stage_vis(const Problem& pr, const PointState<V, SPEC>& st, int s0,
+                                          int sn, bool single, uint32_t* vis, int wv, int ntc,
+                                          int lane) {
+  if (V != kVis) return;
+  const int nsl = (sn + 31) >> 5;
+  for (int sl = wv; sl < nsl; sl += ntc) {
+    const uint32_t* base = pr.vis + (int64_t)((s0 >> 5) + sl) * pr.N_vertex;
+    vis[(sl * 3 + 0) * 64 + lane] = base[st.vert0];
+    if (!single) {
+      vis[(sl * 3 + 1) * 64 + lane] = base[st.vert1];
+      vis[(sl * 3 + 2) * 64 + lane] = base[st.vert2];
+    }
+  }
+}
+
+template <int V>
+__device__ __forceinline__ VisWords staged_words(const uint32_t* vis, int sl, bool single,
+                                                 int lane) {
   VisWords w{0u, 0u, 0u};
-  if (V == kVis && k < ng) {
-    const int tg = group_texel(k, wv, ntc);
-    if (tg < sn) w = load_slot<V, SPEC>(pr, st, (s0 + tg) >> 5, single);
+  if (V == kVis) {
+    w.w0 = vis[(sl * 3 + 0) * 64 + lane];
+    if (!single) {
+      w.w1 = vis[(sl * 3 + 1) * 64 + lane];
+      w.w2 = vis[(sl * 3 + 2) * 64 + lane];
+    }
   }
   return w;
 }
@@ -306,23 +310,22 @@ __device__ __forceinline__ void forward_tile(const Problem& pr, const Tiling& tl
   float sp0 = 0.f, sp1 = 0.f, sp2 = 0.f;  // sum E dOmega vis F G D / 4
 
   Texel* tab = reinterpret_cast<Texel*>(smem);
+  uint32_t* svis = reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(smem) + tl.vis_off);
   for (int s0 = 0; s0 < pr.T; s0 += tl.st) {
     const int sn = min(tl.st, pr.T - s0);
     __syncthreads();
     build_table(pr, s0, sn, tl.st, tab);
+    stage_vis(pr, st, s0, sn, single, svis, wv, tl.ntc, lane);
     __syncthreads();
     const int ng = tl.cw >> 2;
     uint64_t dark = 0ull;
-    // visibility words of the next group are requested one group ahead
-    VisWords wn = load_group_words(pr, st, s0, sn, 0, ng, wv, tl.ntc, single);
     for (int k = 0; k < ng; ++k) {
       const int tg = group_texel(k, wv, tl.ntc);
       if (tg >= sn) break;
-      const VisWords w = wn;
-      wn = load_group_words(pr, st, s0, sn, k + 1, ng, wv, tl.ntc, single);
       if (CULL && (k & 15) == 0) dark = dark_mask(tab, cone, k, ng, wv, tl.ntc, lane);
       const uint32_t d4 = CULL ? (uint32_t)(dark >> (4 * (k & 15))) & 0xfu : 0u;
       if (d4 == 0xfu) continue;  // lights no lane of the tile: adds exactly 0
+      const VisWords w = staged_words<V>(svis, tg >> 5, single, lane);
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         if ((d4 >> i) & 1u) continue;
@@ -437,6 +440,7 @@ __device__ __forceinline__ void backward_tile(const Problem& pr, const Tiling& t
   float Sb = 0.f, ga2 = 0.f, gk = 0.f;
 
   Texel* tab = reinterpret_cast<Texel*>(smem);
+  uint32_t* svis = reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(smem) + tl.vis_off);
   float* part = pr.part + tile * 3 * pr.T;
   // lanes (lane & 15) < 3 of row k store channel (lane & 15) of texel kRowTexel[k]
   const int row = lane >> 4;
@@ -447,16 +451,14 @@ __device__ __forceinline__ void backward_tile(const Problem& pr, const Tiling& t
     const int sn = min(tl.st, pr.T - s0);
     __syncthreads();
     build_table(pr, s0, sn, tl.st, tab);
+    stage_vis(pr, st, s0, sn, single, svis, wv, tl.ntc, lane);
     __syncthreads();
     const int ng = tl.cw >> 2;
     uint64_t dark = 0ull;
-    VisWords wn = load_group_words(pr, st, s0, sn, 0, ng, wv, tl.ntc, single);
 #pragma unroll 1
     for (int k = 0; k < ng; ++k) {
       const int tg = group_texel(k, wv, tl.ntc);
       if (tg >= sn) break;
-      const VisWords w = wn;
-      wn = load_group_words(pr, st, s0, sn, k + 1, ng, wv, tl.ntc, single);
       if (CULL && (k & 15) == 0) dark = dark_mask(tab, cone, k, ng, wv, tl.ntc, lane);
       const uint32_t d4 = CULL ? (uint32_t)(dark >> (4 * (k & 15))) & 0xfu : 0u;
       const int t_out = s0 + tg + row_texel;
@@ -465,6 +467,7 @@ __device__ __forceinline__ void backward_tile(const Problem& pr, const Tiling& t
         if (ch_out < 3 && t_out < pr.T) part[(int64_t)ch_out * pr.T + t_out] = 0.f;
         continue;
       }
+      const VisWords w = staged_words<V>(svis, tg >> 5, single, lane);
       const int b4 = tg & 31;
       {
         float q[4][3];

@@ -16,16 +16,17 @@
 // depend on the order at all: each point's texel sums run in the same order
 // whatever tile it lands in.
 //
-// Blocks of 4096 points (1024 threads, 4 rounds of one point per thread):
+// Blocks of 1024 points, one per thread (small blocks spread the ranking over
+// many CUs):
 //   1. key of each point and its rank among the equal-key points of its block
 //      that precede it (within a wave: a match mask built from one ballot per
 //      key bit; across waves and rounds: per-key counts in LDS), plus the
 //      block's per-key counts, stored key-major: count[key][block];
-//   2. (more than 64 blocks only) per key, an exclusive scan of count[key][.]
+//   2. (more than 8 blocks only) per key, an exclusive scan of count[key][.]
 //      over blocks in place, and the key totals;
 //   3. key bases (an exclusive scan over keys of the totals, in LDS) and the
 //      scatter: each point goes to base[key] + prefix[key][block] + rank.  Up
-//      to 64 blocks, each block sums the count rows itself (step 2 fused).
+//      to 8 blocks, each block sums the count rows itself (step 2 fused).
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -36,11 +37,14 @@ namespace des {
 namespace {
 
 constexpr int kThreads = 1024;
-constexpr int kRounds = kOrderBlock / kThreads;  // 4
+constexpr int kRounds = kOrderBlock / kThreads;  // 1
 constexpr int kWaves = kThreads / 64;             // 16
 constexpr int kKeys = kOrderKeys;                 // 1024
 constexpr int kKeyBits = 10;
-constexpr int kFusedBlocks = 64;                  // fused scan up to 64 blocks (262144 points)
+// Up to this many blocks each scatter block sums the count rows itself (that
+// work grows with the square of the block count); beyond, a separate per-key
+// scan runs first.
+constexpr int kFusedBlocks = 8;
 
 // bits b0..b4 of x -> bit positions 0, 2, 4, 6, 8
 __device__ __forceinline__ uint32_t spread5(uint32_t x) {
@@ -56,7 +60,8 @@ __device__ __forceinline__ uint32_t spread5(uint32_t x) {
 __device__ __forceinline__ uint32_t normal_key(float x, float y, float z) {
   const float s = fabsf(x) + fabsf(y) + fabsf(z);
   if (!(s > 0.f) || !(s < INFINITY)) return 0u;
-  float u = x / s, v = y / s;
+  const float is = __builtin_amdgcn_rcpf(s);  // any deterministic rounding will do
+  float u = x * is, v = y * is;
   if (z < 0.f) {
     const float fu = (1.f - fabsf(v)) * (u < 0.f ? -1.f : 1.f);
     const float fv = (1.f - fabsf(u)) * (v < 0.f ? -1.f : 1.f);
@@ -77,13 +82,15 @@ __global__ __launch_bounds__(kThreads) void order_rank_kernel(const float* __res
                                                               int64_t P, uint32_t* __restrict__ keyrank,
                                                               uint32_t* __restrict__ count, int nblk,
                                                               int nblk_pad) {
-  // wcount[w][k] = (round + 1) << 12 | n: first the number n of key-k points of
+  // wcount[k][w] = (round + 1) << 12 | n: first the number n of key-k points of
   // wave w in that round, then (after the column scan) the number of key-k
-  // points of this block that precede wave w's
-  __shared__ uint16_t wcount[kWaves][kKeys];  // 32 KB
-  __shared__ uint16_t run[kKeys];             // key-k points of this block in earlier rounds
-  static_assert(kKeys == kThreads, "thread k owns key k");
+  // points of this block that precede wave w's.  Key-major, so that thread k
+  // moves its key's 16 entries with two 16-byte LDS accesses.
+  __shared__ __attribute__((aligned(16))) uint16_t wcount[kKeys * kWaves];  // 32 KB
+  __shared__ uint16_t run[kKeys];  // key-k points of this block in earlier rounds
+  static_assert(kKeys == kThreads && kWaves == 16, "thread k owns key k's 16 wave counts");
   const int tid = threadIdx.x;
+  uint16_t* const col = wcount + tid * kWaves;  // 32 bytes, 16-byte aligned
   const int lane = tid & 63;
   const int wv = tid >> 6;
   const int blk = blockIdx.x;
@@ -100,7 +107,9 @@ __global__ __launch_bounds__(kThreads) void order_rank_kernel(const float* __res
       nrm[r][2] = normal[3 * p + 2];
     }
   }
-  for (int i = tid; i < kWaves * kKeys; i += kThreads) (&wcount[0][0])[i] = 0;
+  const uint4 zero = make_uint4(0u, 0u, 0u, 0u);
+  __builtin_memcpy(col, &zero, 16);
+  __builtin_memcpy(col + 8, &zero, 16);
   run[tid] = 0;
   __syncthreads();
   // ranks stay in registers until the end: a barrier waits for outstanding
@@ -121,24 +130,36 @@ __global__ __launch_bounds__(kThreads) void order_rank_kernel(const float* __res
       eq &= bit ? m : ~m;
     }
     const uint32_t rank_w = (uint32_t)__popcll(eq & lanemask_lt(lane));
-    const uint16_t tag = (uint16_t)((r + 1) << 12);
-    if (valid && rank_w == 0u) wcount[wv][key] = (uint16_t)(tag | (uint16_t)__popcll(eq));
+    const uint32_t tag = (uint32_t)(r + 1) << 12;
+    if (valid && rank_w == 0u) wcount[key * kWaves + wv] = (uint16_t)(tag | (uint32_t)__popcll(eq));
     __syncthreads();
-    // column scan of key tid over the waves: counts of this round -> prefixes
+    // column scan of key tid over the waves: this round's counts -> prefixes
+    // (entries of other rounds count 0; every entry is rewritten with this
+    // round's tag, which the next round reads as stale)
     {
+      uint4 c0, c1;
+      __builtin_memcpy(&c0, col, 16);
+      __builtin_memcpy(&c1, col + 8, 16);
+      const uint32_t c[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
       uint32_t acc = run[tid];
+      uint32_t out[8];
 #pragma unroll
-      for (int w = 0; w < kWaves; ++w) {
-        const uint16_t c = wcount[w][tid];
-        if ((c & 0xf000) == tag) {
-          wcount[w][tid] = (uint16_t)(tag | acc);
-          acc += c & 0x0fffu;
-        }
+      for (int i = 0; i < 8; ++i) {
+        const uint32_t lo = c[i] & 0xffffu, hi = c[i] >> 16;
+        const uint32_t plo = acc;
+        acc += ((lo & 0xf000u) == tag) ? (lo & 0x0fffu) : 0u;
+        const uint32_t phi = acc;
+        acc += ((hi & 0xf000u) == tag) ? (hi & 0x0fffu) : 0u;
+        out[i] = (tag | plo) | ((tag | phi) << 16);
       }
+      const uint4 o0 = make_uint4(out[0], out[1], out[2], out[3]);
+      const uint4 o1 = make_uint4(out[4], out[5], out[6], out[7]);
+      __builtin_memcpy(col, &o0, 16);
+      __builtin_memcpy(col + 8, &o1, 16);
       run[tid] = (uint16_t)acc;
     }
     __syncthreads();
-    kr[r] = valid ? ((key << 16) | ((uint32_t)(wcount[wv][key] & 0x0fff) + rank_w)) : 0u;
+    kr[r] = valid ? ((key << 16) | ((uint32_t)(wcount[key * kWaves + wv] & 0x0fffu) + rank_w)) : 0u;
   }
 #pragma unroll
   for (int r = 0; r < kRounds; ++r) {
@@ -217,7 +238,8 @@ __global__ __launch_bounds__(kThreads) void order_scatter_kernel(
   if (lane == 63) wsum[wv] = inc;
   __syncthreads();
   uint32_t o = 0u;
-  for (int w = 0; w < wv; ++w) o += wsum[w];
+#pragma unroll
+  for (int w = 0; w < kWaves; ++w) o += (w < wv) ? wsum[w] : 0u;  // all 16 reads issued at once
   off[tid] = o + inc - tot + pre;
   __syncthreads();
 #pragma unroll
