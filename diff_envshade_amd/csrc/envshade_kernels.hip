@@ -619,8 +619,10 @@ __global__ __launch_bounds__(1024) void shade_forward_kernel(Problem pr, Tiling 
   forward_tile<V, DIFF, SPEC, true>(pr, tl, blockIdx.x);
 }
 
+// Three waves per SIMD (<= 168 VGPRs): the measured best for this kernel.
 template <int V, bool DIFF, bool SPEC>
-__global__ __launch_bounds__(512) void shade_backward_kernel(Problem pr, Tiling tl) {
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(3))) void shade_backward_kernel(
+    Problem pr, Tiling tl) {
   backward_tile<V, DIFF, SPEC, true>(pr, tl, blockIdx.x);
 }
 

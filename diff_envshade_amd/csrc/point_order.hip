@@ -4,10 +4,10 @@
 // with no lit lane: every texel in the opposite hemisphere of all 64 normals
 // has l.n <= 0 on every lane and contributes exactly nothing, so the shading
 // kernels skip it.  With random point order a tile's normals cover the sphere
-// and no texel is ever dead; sorted by normal direction about 45% of the
-// (tile, texel) pairs of a 16x32 map are.
+// and no texel is ever dark; sorted by normal direction about 43% of the
+// (tile, texel) pairs of a 16x32 map are (at least a third in every tile).
 //
-// Key: the 10-bit Morton code of the normal's cell on a 32 x 32 octahedral
+// Key: the 10-bit Hilbert index of the normal's cell on a 32 x 32 octahedral
 // map, so consecutive keys are neighbouring directions.  The order is a
 // stable counting sort by key -- points with equal keys keep their index
 // order -- computed without atomics on positions, so it depends only on the
@@ -46,17 +46,32 @@ constexpr int kKeyBits = 10;
 // scan runs first.
 constexpr int kFusedBlocks = 8;
 
-// bits b0..b4 of x -> bit positions 0, 2, 4, 6, 8
-__device__ __forceinline__ uint32_t spread5(uint32_t x) {
-  x &= 31u;
-  x = (x | (x << 4)) & 0x0f0fu;
-  x = (x | (x << 2)) & 0x3333u;
-  x = (x | (x << 1)) & 0x5555u;
-  return x;
+// Hilbert-curve index of cell (x, y) of a 32 x 32 grid.  Consecutive indices
+// are always neighbouring cells, so a run of 64 sorted points stays compact
+// (a Morton order jumps across the grid at power-of-two boundaries, and a tile
+// straddling such a jump has a wide normal cone and few dark texels).
+__device__ __forceinline__ uint32_t hilbert5(uint32_t x, uint32_t y) {
+  uint32_t d = 0u;
+#pragma unroll
+  for (uint32_t s = 16u; s > 0u; s >>= 1) {
+    const uint32_t rx = (x & s) ? 1u : 0u;
+    const uint32_t ry = (y & s) ? 1u : 0u;
+    d += s * s * ((3u * rx) ^ ry);
+    if (ry == 0u) {
+      if (rx == 1u) {
+        x = 31u - x;
+        y = 31u - y;
+      }
+      const uint32_t t = x;
+      x = y;
+      y = t;
+    }
+  }
+  return d;
 }
 
-// 10-bit Morton code of the 32 x 32 octahedral cell of direction (x, y, z).
-// Zero or non-finite normals get key 0.
+// 10-bit key of direction (x, y, z): the Hilbert index of its cell on a
+// 32 x 32 octahedral map.  Zero or non-finite normals get key 0.
 __device__ __forceinline__ uint32_t normal_key(float x, float y, float z) {
   const float s = fabsf(x) + fabsf(y) + fabsf(z);
   if (!(s > 0.f) || !(s < INFINITY)) return 0u;
@@ -70,7 +85,7 @@ __device__ __forceinline__ uint32_t normal_key(float x, float y, float z) {
   }
   const int iu = min(max((int)((u + 1.f) * 16.f), 0), 31);
   const int iv = min(max((int)((v + 1.f) * 16.f), 0), 31);
-  return spread5((uint32_t)iu) | (spread5((uint32_t)iv) << 1);
+  return hilbert5((uint32_t)iu, (uint32_t)iv);
 }
 
 __device__ __forceinline__ uint64_t lanemask_lt(int lane) {
