@@ -168,7 +168,8 @@ __global__ __launch_bounds__(64) void relight_AB_kernel(RelightArgs a, bool spec
 // ---- forward: 4 waves x 16 points per workgroup, all texels, all columns ------------
 
 template <int PREC, bool SPEC>
-__global__ __launch_bounds__(256) void relight_forward_kernel(RelightArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void relight_forward_kernel(
+    RelightArgs a) {
   constexpr int NF = (PREC == kPrecBF16x3) ? 2 : 1;
   __shared__ uint4 s_frag[kRelightMaxNT * NF * 64];
   __shared__ Texel s_geo[32];
@@ -199,6 +200,9 @@ __global__ __launch_bounds__(256) void relight_forward_kernel(RelightArgs a) {
     for (int j = 0; j < 8; ++j) {
       const float vis = (float)__builtin_amdgcn_ubfe(word, tb + j, 1);
       pair_weights<SPEC>(pc, s_geo[tb + j], vis, w1[j], w2[j]);
+      // one pair at a time: interleaving the eight lobes would hold all their
+      // temporaries at once and cost occupancy
+      __builtin_amdgcn_sched_barrier(0);
     }
     uint4 a1h, a1l, a2h, a2l;
     to_frag<PREC>(w1, a1h, a1l);
@@ -251,7 +255,8 @@ __global__ __launch_bounds__(256) void relight_forward_kernel(RelightArgs a) {
 // ---- env-map gradient: 4 waves x 16 texels per workgroup, one point chunk ----------
 
 template <int PREC, bool SPEC>
-__global__ __launch_bounds__(256) void relight_envgrad_kernel(RelightArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void relight_envgrad_kernel(
+    RelightArgs a) {
   constexpr int NF = (PREC == kPrecBF16x3) ? 2 : 1;
   __shared__ uint4 s_ab[kRelightMaxNT * 2 * NF * 64];
   __shared__ PointGeom s_pc[32];
@@ -293,6 +298,7 @@ __global__ __launch_bounds__(256) void relight_envgrad_kernel(RelightArgs a) {
       const PointGeom& pc = s_pc[pt];
       const float vis = (float)__builtin_amdgcn_ubfe(s_vis[slot_local][pt], tbit, 1);
       pair_weights<SPEC>(pc, g, vis, w1[j], w2[j]);
+      __builtin_amdgcn_sched_barrier(0);  // one pair at a time, as in the forward
     }
     uint4 a1h, a1l, a2h, a2l;
     to_frag<PREC>(w1, a1h, a1l);
