@@ -280,6 +280,46 @@ __device__ __forceinline__ uint64_t dark_mask(const Texel* tab, const Cone& c, i
 
 // ---- forward -----------------------------------------------------------------
 
+// One wave's texel groups of one staged super-chunk (sn texels, ng groups).
+// A group that lights some lane is shaded whole, without a branch per texel,
+// so the four lobes form one block the compiler can interleave; its dark
+// texels add exact zeros (see Cone).  SINGLE: the wave-uniform case of one
+// visibility word per slot, chosen once per super-chunk.
+// acc = {sum E vis max(l.n,0) per channel, sum E vis F G D / 4 per channel}.
+template <int V, bool DIFF, bool SPEC, bool CULL, bool SINGLE>
+__device__ __forceinline__ void forward_chunk(const PointState<V, SPEC>& st, const Texel* tab,
+                                              const uint32_t* svis, const Cone& cone, int sn,
+                                              int ng, int wv, int ntc, int lane, float csum,
+                                              float (&acc)[6]) {
+  uint64_t dark = 0ull;
+  for (int k = 0; k < ng; ++k) {
+    const int tg = group_texel(k, wv, ntc);
+    if (tg >= sn) break;
+    if (CULL && (k & 15) == 0) dark = dark_mask(tab, cone, k, ng, wv, ntc, lane);
+    const uint32_t d4 = CULL ? (uint32_t)(dark >> (4 * (k & 15))) & 0xfu : 0u;
+    if (d4 == 0xfu) continue;  // lights no lane of the tile: adds exactly 0
+    const VisWords w = staged_words<V>(svis, tg >> 5, SINGLE, lane);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const Texel& g = tab[tg + i];
+      const float vis = vis_weight<V, SPEC>(st, w, (tg & 31) + i, SINGLE, csum);
+      const Lobe lb = eval_lobe<SPEC>(st, g);
+      if (DIFF) {
+        const float x = lb.nl * vis;
+        acc[0] = fmaf(g.e0, x, acc[0]);
+        acc[1] = fmaf(g.e1, x, acc[1]);
+        acc[2] = fmaf(g.e2, x, acc[2]);
+      }
+      if (SPEC) {
+        const float y = lb.tmp * vis;
+        acc[3] = fmaf(g.e0, y, acc[3]);
+        acc[4] = fmaf(g.e1, y, acc[4]);
+        acc[5] = fmaf(g.e2, y, acc[5]);
+      }
+    }
+  }
+}
+
 // One 64-point tile of one problem; called by every wave of the workgroup.
 // Waves other than wave 0 return after the cross-chunk reduction; wave 0
 // writes the outputs.  No barrier follows the last one executed here, so a
@@ -306,8 +346,8 @@ __device__ __forceinline__ void forward_tile(const Problem& pr, const Tiling& tl
   Cone cone{0.f, 0.f, 0.f, 4.f};
   if (CULL) cone = tile_cone(st);
 
-  float ds0 = 0.f, ds1 = 0.f, ds2 = 0.f;  // sum E dOmega vis max(l.n,0)
-  float sp0 = 0.f, sp1 = 0.f, sp2 = 0.f;  // sum E dOmega vis F G D / 4
+  constexpr int NACC = kForwardAcc;
+  float acc[NACC] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
 
   Texel* tab = reinterpret_cast<Texel*>(smem);
   uint32_t* svis = reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(smem) + tl.vis_off);
@@ -318,39 +358,13 @@ __device__ __forceinline__ void forward_tile(const Problem& pr, const Tiling& tl
     stage_vis(pr, st, s0, sn, single, svis, wv, tl.ntc, lane);
     __syncthreads();
     const int ng = tl.cw >> 2;
-    uint64_t dark = 0ull;
-    for (int k = 0; k < ng; ++k) {
-      const int tg = group_texel(k, wv, tl.ntc);
-      if (tg >= sn) break;
-      if (CULL && (k & 15) == 0) dark = dark_mask(tab, cone, k, ng, wv, tl.ntc, lane);
-      const uint32_t d4 = CULL ? (uint32_t)(dark >> (4 * (k & 15))) & 0xfu : 0u;
-      if (d4 == 0xfu) continue;  // lights no lane of the tile: adds exactly 0
-      const VisWords w = staged_words<V>(svis, tg >> 5, single, lane);
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        if ((d4 >> i) & 1u) continue;
-        const Texel& g = tab[tg + i];
-        const float vis = vis_weight<V, SPEC>(st, w, (tg & 31) + i, single, csum);
-        const Lobe lb = eval_lobe<SPEC>(st, g);
-        if (DIFF) {
-          const float x = lb.nl * vis;
-          ds0 = fmaf(g.e0, x, ds0);
-          ds1 = fmaf(g.e1, x, ds1);
-          ds2 = fmaf(g.e2, x, ds2);
-        }
-        if (SPEC) {
-          const float y = lb.tmp * vis;
-          sp0 = fmaf(g.e0, y, sp0);
-          sp1 = fmaf(g.e1, y, sp1);
-          sp2 = fmaf(g.e2, y, sp2);
-        }
-      }
-    }
+    if (single)
+      forward_chunk<V, DIFF, SPEC, CULL, true>(st, tab, svis, cone, sn, ng, wv, tl.ntc, lane, csum, acc);
+    else
+      forward_chunk<V, DIFF, SPEC, CULL, false>(st, tab, svis, cone, sn, ng, wv, tl.ntc, lane, csum, acc);
   }
 
   // fixed-order reduction over the NTC texel chunks
-  constexpr int NACC = 6;
-  float acc[NACC] = {ds0, ds1, ds2, sp0, sp1, sp2};
   if (tl.ntc > 1) {
     __syncthreads();
     float* red = reinterpret_cast<float*>(smem);
@@ -380,6 +394,138 @@ __device__ __forceinline__ void forward_tile(const Problem& pr, const Tiling& tl
   if (DOUT) {
 #pragma unroll
     for (int c = 0; c < 3; ++c) pr.ds[3 * p + c] = dsh[c];
+  }
+}
+
+// Per-point texel sums of the backward (one wave's share of the texels).
+struct BwdSums {
+  float ds0, ds1, ds2;  // sum E vis max(l.n,0)
+  float sp0, sp1, sp2;  // sum E vis F G D / 4
+  float lnx, lny, lnz;  // sum coef_n l
+  float lvx, lvy, lvz;  // sum coef_v l
+  float hnx, hny, hnz;  // sum rs gdnh (l + v)
+  float hvx, hvy, hvz;  // sum rs^2 gch (l.h) (l + v)
+  float cvx, cvy, cvz;  // sum rs^3 gdnh (hb x (n x hb))
+  float Sb, ga2, gk;
+};
+
+// One wave's texel groups of one staged super-chunk at s0 (sn texels, ng
+// groups): per-point sums into a, and per group the tile's env-map partials of
+// its 4 texels into part.  Unlike the forward, dark texels of a lit group are
+// skipped one by one: four backward lobes in one block need more than the
+// 168 registers that keep three waves per SIMD.
+template <int V, bool DIFF, bool SPEC, bool CULL>
+__device__ __forceinline__ void backward_chunk(const PointState<V, SPEC>& st, const Texel* tab,
+                                               const uint32_t* svis, const Cone& cone, int s0,
+                                               int sn, int ng, int wv, int ntc, int lane,
+                                               bool single, float csum, const float* A,
+                                               const float* B, int T, float* part, int row_texel,
+                                               int ch_out, BwdSums& a) {
+  uint64_t dark = 0ull;
+#pragma unroll 1
+  for (int k = 0; k < ng; ++k) {
+    const int tg = group_texel(k, wv, ntc);
+    if (tg >= sn) break;
+    if (CULL && (k & 15) == 0) dark = dark_mask(tab, cone, k, ng, wv, ntc, lane);
+    const uint32_t d4 = CULL ? (uint32_t)(dark >> (4 * (k & 15))) & 0xfu : 0u;
+    const int t_out = s0 + tg + row_texel;
+    if (d4 == 0xfu) {
+      // the 4 texels light no lane of the tile: their partials are exactly 0
+      if (ch_out < 3 && t_out < T) part[(int64_t)ch_out * T + t_out] = 0.f;
+      continue;
+    }
+    const VisWords w = staged_words<V>(svis, tg >> 5, single, lane);
+    const int b4 = tg & 31;
+    {
+      float q[4][3];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        q[i][0] = q[i][1] = q[i][2] = 0.f;
+        if ((d4 >> i) & 1u) continue;
+        const Texel& g = tab[tg + i];
+        const float vis = vis_weight<V, SPEC>(st, w, b4 + i, single, csum);
+        const Lobe lb = eval_lobe<SPEC>(st, g);
+        float coef_n = 0.f;
+        float x = 0.f, y = 0.f;
+        if (DIFF) {
+          x = lb.nl * vis;
+          a.ds0 = fmaf(g.e0, x, a.ds0);
+          a.ds1 = fmaf(g.e1, x, a.ds1);
+          a.ds2 = fmaf(g.e2, x, a.ds2);
+          // gdot = vis [l.n > 0] sum_c A_c E'_c   (bs:2426-2438)
+          const float AE = fmaf(A[2], g.e2, fmaf(A[1], g.e1, A[0] * g.e0));
+          coef_n = (lb.cs > 0.f) ? AE * vis : 0.f;
+        }
+        if (SPEC) {
+          y = lb.tmp * vis;
+          a.sp0 = fmaf(g.e0, y, a.sp0);
+          a.sp1 = fmaf(g.e1, y, a.sp1);
+          a.sp2 = fmaf(g.e2, y, a.sp2);
+          const float BE = fmaf(B[2], g.e2, fmaf(B[1], g.e1, B[0] * g.e0));
+          const float gt = lb.spec_on ? BE * vis : 0.f;  // grad of F G D / 4
+          const float gq = 0.25f * gt;
+          const float gF = gq * lb.G * lb.D;
+          const float gG = gq * lb.F * lb.D;
+          const float gD = gq * lb.F * lb.G;
+          // Fresnel VJP (bs:155-159): 5 (F0-1) g (1-sat c)^4 [0<=c<=1]
+          const float gch = (lb.ch >= 0.f && lb.ch <= 1.f) ? (5.f * (kF0 - 1.f)) * gF * lb.om4 : 0.f;
+          // G1 VJP (bs:173-201); l.n > 0 holds wherever gt != 0
+          const float grad_a = gG * st.ib;
+          const float grad_b = -gG * lb.G * st.ib;
+          const float ia2 = lb.ia * lb.ia;
+          const float gdl = grad_a * st.k * ia2;
+          a.gk = fmaf(grad_a * lb.nl * (lb.nl - 1.f), ia2, a.gk);
+          a.Sb += grad_b;
+          // NDF VJP (bs:212-227)
+          a.ga2 = fmaf(gD, lb.it2, a.ga2);
+          const bool big = kPi * lb.tD * lb.tD > kThres;
+          const float g_t = big ? -2.f * gD * lb.D * rcp(lb.tD) : 0.f;
+          const float nh2 = lb.nh * lb.nh;
+          a.ga2 = fmaf(nh2, g_t, a.ga2);
+          const float gdnh = 2.f * g_t * st.am1 * lb.nh;
+          // h = hb rs with hb = l + v; grad_h = gch l + gdnh n.
+          //   NDF term of grad_normal:   gdnh h = (rs gdnh) hb
+          //   normalize VJP (bs:62-71):  rs (grad_h - h (h.grad_h))
+          //     = rs gch (l - (l.h) h) + rs gdnh (n - (n.h) h)
+          //     = rs gch l - rs^2 gch (l.h) hb + rs^3 gdnh (hb x (n x hb))
+          // The last form of n - (n.h) h (size sin(theta_h)) avoids the
+          // 1/sin(theta_h) cancellation of sharp lobes; hb is accumulated as a
+          // vector because splitting it into l and v parts would cancel by
+          // ~1/|l+v| when l and v are nearly opposite.
+          const float rg = lb.rs * gdnh;
+          const float rs2 = lb.rs * lb.rs;
+          const float q2 = rs2 * gch * lb.ch;
+          const float r3 = rs2 * rg;
+          coef_n += gdl;
+          a.hnx = fmaf(rg, lb.hx, a.hnx);
+          a.hny = fmaf(rg, lb.hy, a.hny);
+          a.hnz = fmaf(rg, lb.hz, a.hnz);
+          a.hvx = fmaf(q2, lb.hx, a.hvx);
+          a.hvy = fmaf(q2, lb.hy, a.hvy);
+          a.hvz = fmaf(q2, lb.hz, a.hvz);
+          a.cvx = fmaf(r3, lb.ux, a.cvx);
+          a.cvy = fmaf(r3, lb.uy, a.cvy);
+          a.cvz = fmaf(r3, lb.uz, a.cvz);
+          const float coef_v = lb.rs * gch;
+          a.lvx = fmaf(coef_v, g.lx, a.lvx);
+          a.lvy = fmaf(coef_v, g.ly, a.lvy);
+          a.lvz = fmaf(coef_v, g.lz, a.lvz);
+        }
+        a.lnx = fmaf(coef_n, g.lx, a.lnx);
+        a.lny = fmaf(coef_n, g.ly, a.lny);
+        a.lnz = fmaf(coef_n, g.lz, a.lnz);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) q[i][c] = fmaf(A[c], x, B[c] * y);
+      }
+      // env-map gradient: sum over the 64 points of this tile, 4 texels at a time
+      const float r0 = reduce4(q[0][0], q[1][0], q[2][0], q[3][0]);
+      const float r1 = reduce4(q[0][1], q[1][1], q[2][1], q[3][1]);
+      const float r2 = reduce4(q[0][2], q[1][2], q[2][2], q[3][2]);
+      if (ch_out < 3 && t_out < T) {
+        const float val = (ch_out == 0) ? r0 : ((ch_out == 1) ? r1 : r2);
+        part[(int64_t)ch_out * T + t_out] = val;
+      }
+    }
   }
 }
 
@@ -430,14 +576,7 @@ __device__ __forceinline__ void backward_tile(const Problem& pr, const Tiling& t
   Cone cone{0.f, 0.f, 0.f, 4.f};
   if (CULL) cone = tile_cone(st);
 
-  float ds0 = 0.f, ds1 = 0.f, ds2 = 0.f;
-  float sp0 = 0.f, sp1 = 0.f, sp2 = 0.f;
-  float lnx = 0.f, lny = 0.f, lnz = 0.f;  // sum coef_n l
-  float lvx = 0.f, lvy = 0.f, lvz = 0.f;  // sum coef_v l
-  float hnx = 0.f, hny = 0.f, hnz = 0.f;  // sum rs gdnh (l + v)
-  float hvx = 0.f, hvy = 0.f, hvz = 0.f;  // sum rs^2 gch (l.h) (l + v)
-  float cvx = 0.f, cvy = 0.f, cvz = 0.f;  // sum rs^3 gdnh (hb x (n x hb))
-  float Sb = 0.f, ga2 = 0.f, gk = 0.f;
+  BwdSums a{};
 
   Texel* tab = reinterpret_cast<Texel*>(smem);
   uint32_t* svis = reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(smem) + tl.vis_off);
@@ -454,118 +593,15 @@ __device__ __forceinline__ void backward_tile(const Problem& pr, const Tiling& t
     stage_vis(pr, st, s0, sn, single, svis, wv, tl.ntc, lane);
     __syncthreads();
     const int ng = tl.cw >> 2;
-    uint64_t dark = 0ull;
-#pragma unroll 1
-    for (int k = 0; k < ng; ++k) {
-      const int tg = group_texel(k, wv, tl.ntc);
-      if (tg >= sn) break;
-      if (CULL && (k & 15) == 0) dark = dark_mask(tab, cone, k, ng, wv, tl.ntc, lane);
-      const uint32_t d4 = CULL ? (uint32_t)(dark >> (4 * (k & 15))) & 0xfu : 0u;
-      const int t_out = s0 + tg + row_texel;
-      if (d4 == 0xfu) {
-        // the 4 texels light no lane of the tile: their partials are exactly 0
-        if (ch_out < 3 && t_out < pr.T) part[(int64_t)ch_out * pr.T + t_out] = 0.f;
-        continue;
-      }
-      const VisWords w = staged_words<V>(svis, tg >> 5, single, lane);
-      const int b4 = tg & 31;
-      {
-        float q[4][3];
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          q[i][0] = q[i][1] = q[i][2] = 0.f;
-          if ((d4 >> i) & 1u) continue;
-          const Texel& g = tab[tg + i];
-          const float vis = vis_weight<V, SPEC>(st, w, b4 + i, single, csum);
-          const Lobe lb = eval_lobe<SPEC>(st, g);
-          float coef_n = 0.f;
-          float x = 0.f, y = 0.f;
-          if (DIFF) {
-            x = lb.nl * vis;
-            ds0 = fmaf(g.e0, x, ds0);
-            ds1 = fmaf(g.e1, x, ds1);
-            ds2 = fmaf(g.e2, x, ds2);
-            // gdot = vis [l.n > 0] sum_c A_c E'_c   (bs:2426-2438)
-            const float AE = fmaf(A[2], g.e2, fmaf(A[1], g.e1, A[0] * g.e0));
-            coef_n = (lb.cs > 0.f) ? AE * vis : 0.f;
-          }
-          if (SPEC) {
-            y = lb.tmp * vis;
-            sp0 = fmaf(g.e0, y, sp0);
-            sp1 = fmaf(g.e1, y, sp1);
-            sp2 = fmaf(g.e2, y, sp2);
-            const float BE = fmaf(B[2], g.e2, fmaf(B[1], g.e1, B[0] * g.e0));
-            const float gt = lb.spec_on ? BE * vis : 0.f;  // grad of F G D / 4
-            const float gq = 0.25f * gt;
-            const float gF = gq * lb.G * lb.D;
-            const float gG = gq * lb.F * lb.D;
-            const float gD = gq * lb.F * lb.G;
-            // Fresnel VJP (bs:155-159): 5 (F0-1) g (1-sat c)^4 [0<=c<=1]
-            const float gch = (lb.ch >= 0.f && lb.ch <= 1.f) ? (5.f * (kF0 - 1.f)) * gF * lb.om4 : 0.f;
-            // G1 VJP (bs:173-201); l.n > 0 holds wherever gt != 0
-            const float grad_a = gG * st.ib;
-            const float grad_b = -gG * lb.G * st.ib;
-            const float ia2 = lb.ia * lb.ia;
-            const float gdl = grad_a * st.k * ia2;
-            gk = fmaf(grad_a * lb.nl * (lb.nl - 1.f), ia2, gk);
-            Sb += grad_b;
-            // NDF VJP (bs:212-227)
-            ga2 = fmaf(gD, lb.it2, ga2);
-            const bool big = kPi * lb.tD * lb.tD > kThres;
-            const float g_t = big ? -2.f * gD * lb.D * rcp(lb.tD) : 0.f;
-            const float nh2 = lb.nh * lb.nh;
-            ga2 = fmaf(nh2, g_t, ga2);
-            const float gdnh = 2.f * g_t * st.am1 * lb.nh;
-            // h = hb rs with hb = l + v; grad_h = gch l + gdnh n.
-            //   NDF term of grad_normal:   gdnh h = (rs gdnh) hb
-            //   normalize VJP (bs:62-71):  rs (grad_h - h (h.grad_h))
-            //     = rs gch (l - (l.h) h) + rs gdnh (n - (n.h) h)
-            //     = rs gch l - rs^2 gch (l.h) hb + rs^3 gdnh (hb x (n x hb))
-            // The last form of n - (n.h) h (size sin(theta_h)) avoids the
-            // 1/sin(theta_h) cancellation of sharp lobes; hb is accumulated as a
-            // vector because splitting it into l and v parts would cancel by
-            // ~1/|l+v| when l and v are nearly opposite.
-            const float rg = lb.rs * gdnh;
-            const float rs2 = lb.rs * lb.rs;
-            const float q2 = rs2 * gch * lb.ch;
-            const float r3 = rs2 * rg;
-            coef_n += gdl;
-            hnx = fmaf(rg, lb.hx, hnx);
-            hny = fmaf(rg, lb.hy, hny);
-            hnz = fmaf(rg, lb.hz, hnz);
-            hvx = fmaf(q2, lb.hx, hvx);
-            hvy = fmaf(q2, lb.hy, hvy);
-            hvz = fmaf(q2, lb.hz, hvz);
-            cvx = fmaf(r3, lb.ux, cvx);
-            cvy = fmaf(r3, lb.uy, cvy);
-            cvz = fmaf(r3, lb.uz, cvz);
-            const float coef_v = lb.rs * gch;
-            lvx = fmaf(coef_v, g.lx, lvx);
-            lvy = fmaf(coef_v, g.ly, lvy);
-            lvz = fmaf(coef_v, g.lz, lvz);
-          }
-          lnx = fmaf(coef_n, g.lx, lnx);
-          lny = fmaf(coef_n, g.ly, lny);
-          lnz = fmaf(coef_n, g.lz, lnz);
-#pragma unroll
-          for (int c = 0; c < 3; ++c) q[i][c] = fmaf(A[c], x, B[c] * y);
-        }
-        // env-map gradient: sum over the 64 points of this tile, 4 texels at a time
-        const float r0 = reduce4(q[0][0], q[1][0], q[2][0], q[3][0]);
-        const float r1 = reduce4(q[0][1], q[1][1], q[2][1], q[3][1]);
-        const float r2 = reduce4(q[0][2], q[1][2], q[2][2], q[3][2]);
-        if (ch_out < 3 && t_out < pr.T) {
-          const float val = (ch_out == 0) ? r0 : ((ch_out == 1) ? r1 : r2);
-          part[(int64_t)ch_out * pr.T + t_out] = val;
-        }
-      }
-    }
+    backward_chunk<V, DIFF, SPEC, CULL>(st, tab, svis, cone, s0, sn, ng, wv, tl.ntc, lane, single,
+                                        csum, A, B, pr.T, part, row_texel, ch_out, a);
   }
 
   // fixed-order reduction of the per-point sums over the NTC texel chunks
   constexpr int NACC = kBackwardAcc;
-  float acc[NACC] = {ds0, ds1, ds2, sp0, sp1, sp2, lnx, lny, lnz, lvx, lvy, lvz,
-                     hnx, hny, hnz, hvx, hvy, hvz, cvx, cvy, cvz, Sb,  ga2, gk};
+  float acc[NACC] = {a.ds0, a.ds1, a.ds2, a.sp0, a.sp1, a.sp2, a.lnx, a.lny,
+                     a.lnz, a.lvx, a.lvy, a.lvz, a.hnx, a.hny, a.hnz, a.hvx,
+                     a.hvy, a.hvz, a.cvx, a.cvy, a.cvz, a.Sb,  a.ga2, a.gk};
   if (tl.ntc > 1) {
     __syncthreads();
     float* red = reinterpret_cast<float*>(smem);
