@@ -99,7 +99,7 @@ hipError_t launch_batch_backward(int variant, bool diff, bool spec, const BatchL
 int batch_blocks_per_cu(int variant, bool diff, bool spec, bool fwd, int ntc, size_t lds);
 
 constexpr int kForwardAcc = 6;
-constexpr int kBackwardAcc = 24;
+constexpr int kBackwardAcc = 16;
 constexpr int kMaxSuperChunk = 1024;  // texels staged in LDS at once (48 B each, 48 KB)
 constexpr int kTexelBytes = 48;
 constexpr int kReduceRows = 16;

@@ -401,12 +401,11 @@ __device__ __forceinline__ void forward_tile(const Problem& pr, const Tiling& tl
 struct BwdSums {
   float ds0, ds1, ds2;  // sum E vis max(l.n,0)
   float sp0, sp1, sp2;  // sum E vis F G D / 4
-  float lnx, lny, lnz;  // sum coef_n l
-  float lvx, lvy, lvz;  // sum coef_v l
-  float hnx, hny, hnz;  // sum rs gdnh (l + v)
-  float hvx, hvy, hvz;  // sum rs^2 gch (l.h) (l + v)
-  float cvx, cvy, cvz;  // sum rs^3 gdnh (hb x (n x hb))
-  float Sb, ga2, gk;
+  float gnx, gny, gnz;  // sum coef_n l + rs gdnh hb
+  float gvx, gvy, gvz;  // sum rs gch l - rs^2 gch (l.h) hb + rs^3 gdnh (hb x (n x hb))
+  float Sq;             // sum grad_a G / 4  (the G1(v) term: Sb = -4 Sq)
+  float ga2a, ga2b;     // alpha^2 gradient: sum pi gD it2 and sum (n.h)^2 t
+  float gk;             // sum grad_a nl (nl - 1) ia^2
 };
 
 // One wave's texel groups of one staged super-chunk at s0 (sn texels, ng
@@ -462,58 +461,51 @@ __device__ __forceinline__ void backward_chunk(const PointState<V, SPEC>& st, co
           a.sp1 = fmaf(g.e1, y, a.sp1);
           a.sp2 = fmaf(g.e2, y, a.sp2);
           const float BE = fmaf(B[2], g.e2, fmaf(B[1], g.e1, B[0] * g.e0));
-          const float gt = lb.spec_on ? BE * vis : 0.f;  // grad of F G D / 4
-          const float gq = 0.25f * gt;
-          const float gF = gq * lb.G * lb.D;
-          const float gG = gq * lb.F * lb.D;
-          const float gD = gq * lb.F * lb.G;
-          // Fresnel VJP (bs:155-159): 5 (F0-1) g (1-sat c)^4 [0<=c<=1]
-          const float gch = (lb.ch >= 0.f && lb.ch <= 1.f) ? (5.f * (kF0 - 1.f)) * gF * lb.om4 : 0.f;
-          // G1 VJP (bs:173-201); l.n > 0 holds wherever gt != 0
-          const float grad_a = gG * st.ib;
-          const float grad_b = -gG * lb.G * st.ib;
-          const float ia2 = lb.ia * lb.ia;
-          const float gdl = grad_a * st.k * ia2;
-          a.gk = fmaf(grad_a * lb.nl * (lb.nl - 1.f), ia2, a.gk);
-          a.Sb += grad_b;
-          // NDF VJP (bs:212-227)
-          a.ga2 = fmaf(gD, lb.it2, a.ga2);
-          const bool big = kPi * lb.tD * lb.tD > kThres;
-          const float g_t = big ? -2.f * gD * lb.D * rcp(lb.tD) : 0.f;
-          const float nh2 = lb.nh * lb.nh;
-          a.ga2 = fmaf(nh2, g_t, a.ga2);
-          const float gdnh = 2.f * g_t * st.am1 * lb.nh;
+          const float gt = lb.spec_on ? BE * vis : 0.f;  // grad of tmp = F G D / 4
+          const float PF = gt * lb.F;
+          const float gF = gt * (lb.Gq * lb.D);
+          const float gD = PF * lb.Gq;
+          // G1 VJP (bs:173-201), with grad_a = gG ib = (gt F D / 4) ib and
+          // grad_b = -gG G ib = -4 grad_a G/4; l.n > 0 holds wherever gt != 0
+          const float grad_a = (PF * lb.D) * st.ib4;
+          const float w = grad_a * (lb.ia * lb.ia);
+          const float gdl = w * st.k;
+          a.gk = fmaf(w, fmaf(lb.nl, lb.nl, -lb.nl), a.gk);
+          a.Sq = fmaf(grad_a, lb.Gq, a.Sq);
+          // Fresnel VJP (bs:155-159): 5 (F0-1) gF (1-sat c)^4 where 0 <= c <= 1
+          const float gch = (lb.sch == lb.ch) ? (5.f * (kF0 - 1.f)) * gF * lb.om4 : 0.f;
+          // NDF VJP (bs:212-227): D = a2 / (pi tD^2) gives dD/da2 = it2 and
+          // dD/dtD = -2 D / tD = -2 D tD itD2 (1/tD = tD itD2 where the floor is
+          // not hit; there the reference's gradient through tD is 0)
+          const float gDi = gD * lb.itD2;  // pi gD it2
+          a.ga2a += gDi;
+          const float t = (lb.tD2 > kThresOverPi) ? gDi * (lb.D * lb.tD) : 0.f;  // g_t = -2 t
+          a.ga2b = fmaf(lb.nh * lb.nh, t, a.ga2b);
+          const float gdnh = t * lb.nh * st.c4;  // 2 g_t (alpha^2 - 1) n.h
           // h = hb rs with hb = l + v; grad_h = gch l + gdnh n.
           //   NDF term of grad_normal:   gdnh h = (rs gdnh) hb
           //   normalize VJP (bs:62-71):  rs (grad_h - h (h.grad_h))
           //     = rs gch (l - (l.h) h) + rs gdnh (n - (n.h) h)
           //     = rs gch l - rs^2 gch (l.h) hb + rs^3 gdnh (hb x (n x hb))
           // The last form of n - (n.h) h (size sin(theta_h)) avoids the
-          // 1/sin(theta_h) cancellation of sharp lobes; hb is accumulated as a
-          // vector because splitting it into l and v parts would cancel by
-          // ~1/|l+v| when l and v are nearly opposite.
+          // 1/sin(theta_h) cancellation of sharp lobes; hb is kept as a vector
+          // because splitting it into l and v parts would cancel by ~1/|l+v|
+          // when l and v are nearly opposite.
           const float rg = lb.rs * gdnh;
-          const float rs2 = lb.rs * lb.rs;
-          const float q2 = rs2 * gch * lb.ch;
-          const float r3 = rs2 * rg;
-          coef_n += gdl;
-          a.hnx = fmaf(rg, lb.hx, a.hnx);
-          a.hny = fmaf(rg, lb.hy, a.hny);
-          a.hnz = fmaf(rg, lb.hz, a.hnz);
-          a.hvx = fmaf(q2, lb.hx, a.hvx);
-          a.hvy = fmaf(q2, lb.hy, a.hvy);
-          a.hvz = fmaf(q2, lb.hz, a.hvz);
-          a.cvx = fmaf(r3, lb.ux, a.cvx);
-          a.cvy = fmaf(r3, lb.uy, a.cvy);
-          a.cvz = fmaf(r3, lb.uz, a.cvz);
+          const float q2 = lb.rs2 * gch * lb.ch;
+          const float r3 = lb.rs2 * rg;
           const float coef_v = lb.rs * gch;
-          a.lvx = fmaf(coef_v, g.lx, a.lvx);
-          a.lvy = fmaf(coef_v, g.ly, a.lvy);
-          a.lvz = fmaf(coef_v, g.lz, a.lvz);
+          coef_n += gdl;
+          a.gnx = fmaf(rg, lb.hx, a.gnx);
+          a.gny = fmaf(rg, lb.hy, a.gny);
+          a.gnz = fmaf(rg, lb.hz, a.gnz);
+          a.gvx = fmaf(r3, lb.ux, fmaf(-q2, lb.hx, fmaf(coef_v, g.lx, a.gvx)));
+          a.gvy = fmaf(r3, lb.uy, fmaf(-q2, lb.hy, fmaf(coef_v, g.ly, a.gvy)));
+          a.gvz = fmaf(r3, lb.uz, fmaf(-q2, lb.hz, fmaf(coef_v, g.lz, a.gvz)));
         }
-        a.lnx = fmaf(coef_n, g.lx, a.lnx);
-        a.lny = fmaf(coef_n, g.ly, a.lny);
-        a.lnz = fmaf(coef_n, g.lz, a.lnz);
+        a.gnx = fmaf(coef_n, g.lx, a.gnx);
+        a.gny = fmaf(coef_n, g.ly, a.gny);
+        a.gnz = fmaf(coef_n, g.lz, a.gnz);
 #pragma unroll
         for (int c = 0; c < 3; ++c) q[i][c] = fmaf(A[c], x, B[c] * y);
       }
@@ -599,9 +591,8 @@ __device__ __forceinline__ void backward_tile(const Problem& pr, const Tiling& t
 
   // fixed-order reduction of the per-point sums over the NTC texel chunks
   constexpr int NACC = kBackwardAcc;
-  float acc[NACC] = {a.ds0, a.ds1, a.ds2, a.sp0, a.sp1, a.sp2, a.lnx, a.lny,
-                     a.lnz, a.lvx, a.lvy, a.lvz, a.hnx, a.hny, a.hnz, a.hvx,
-                     a.hvy, a.hvz, a.cvx, a.cvy, a.cvz, a.Sb,  a.ga2, a.gk};
+  float acc[NACC] = {a.ds0, a.ds1, a.ds2, a.sp0, a.sp1, a.sp2, a.gnx, a.gny,
+                     a.gnz, a.gvx, a.gvy, a.gvz, a.Sq,  a.ga2a, a.ga2b, a.gk};
   if (tl.ntc > 1) {
     __syncthreads();
     float* red = reinterpret_cast<float*>(smem);
@@ -617,30 +608,30 @@ __device__ __forceinline__ void backward_tile(const Problem& pr, const Tiling& t
     }
   }
   if (!valid) return;
-  // gn = sum coef_n l + sum rs gdnh (l+v) + v [n.v>0](1-k) Sb
-  // gv = sum rs gch l - sum rs^2 gch (l.h) (l+v) + sum rs^3 gdnh (hb x (n x hb))
+  // gn = sum (coef_n l + rs gdnh hb) + v [n.v>0](1-k) Sb
+  // gv = sum (rs gch l - rs^2 gch (l.h) hb + rs^3 gdnh (hb x (n x hb)))
   //      + n [n.v>0](1-k) Sb
+  const float Sb = -4.f * acc[12];
   const float nvpos = (st.nv > 0.f) ? 1.f : 0.f;
-  const float g1v = SPEC ? nvpos * st.omk * acc[21] : 0.f;
-  pr.gn[3 * p + 0] = fmaf(st.vx, g1v, acc[6] + acc[12]);
-  pr.gn[3 * p + 1] = fmaf(st.vy, g1v, acc[7] + acc[13]);
-  pr.gn[3 * p + 2] = fmaf(st.vz, g1v, acc[8] + acc[14]);
+  const float g1v = SPEC ? nvpos * st.omk * Sb : 0.f;
+  pr.gn[3 * p + 0] = fmaf(st.vx, g1v, acc[6]);
+  pr.gn[3 * p + 1] = fmaf(st.vy, g1v, acc[7]);
+  pr.gn[3 * p + 2] = fmaf(st.vz, g1v, acc[8]);
   if (V != kDiffuse) {
     // disabled branches are written as zeros (bs:2889-2905, bs:3205-3214)
-    const float gvx = fmaf(st.nx, g1v, (acc[9] - acc[15]) + acc[18]);
-    const float gvy = fmaf(st.ny, g1v, (acc[10] - acc[16]) + acc[19]);
-    const float gvz = fmaf(st.nz, g1v, (acc[11] - acc[17]) + acc[20]);
-    pr.gv[3 * p + 0] = SPEC ? gvx : 0.f;
-    pr.gv[3 * p + 1] = SPEC ? gvy : 0.f;
-    pr.gv[3 * p + 2] = SPEC ? gvz : 0.f;
+    pr.gv[3 * p + 0] = SPEC ? fmaf(st.nx, g1v, acc[9]) : 0.f;
+    pr.gv[3 * p + 1] = SPEC ? fmaf(st.ny, g1v, acc[10]) : 0.f;
+    pr.gv[3 * p + 2] = SPEC ? fmaf(st.nz, g1v, acc[11]) : 0.f;
 #pragma unroll
     for (int c = 0; c < 3; ++c) pr.gs[3 * p + c] = SPEC ? gs[c] * acc[3 + c] : 0.f;
     float gr = 0.f;
     if (SPEC) {
       const float r = pr.rough[p];
-      const float gkt = fmaf(1.f - fmaxf(st.nv, 0.f), acc[21], acc[23]);
+      const float gkt = fmaf(1.f - fmaxf(st.nv, 0.f), Sb, acc[15]);
+      // grad of alpha^2: sum gD it2 + sum (n.h)^2 g_t = ga2a / pi - 2 ga2b
+      const float ga2 = fmaf(acc[13], kInvPi, -2.f * acc[14]);
       // bs:2501-2502
-      gr = fmaf(acc[22] * 4.f * r * r, r, gkt * (1.f + r) * 0.25f);
+      gr = fmaf(ga2 * 4.f * r * r, r, gkt * (1.f + r) * 0.25f);
     }
     pr.gr[p] = gr;
   }

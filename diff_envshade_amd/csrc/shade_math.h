@@ -15,6 +15,8 @@ constexpr float kF0 = 0.04f;      // Schlick R0 (bs:30)
 constexpr float kThres = 1e-9f;   // NDF denominator floor (bs:31)
 constexpr float kInvPi = 0.318309886183790671538f;
 constexpr float kPi = 3.14159265358979323846f;
+// the floor on pi tD^2 restated as a floor on tD^2
+constexpr float kThresOverPi = 3.18309886183790671538e-10f;
 constexpr double kPiD = 3.14159265358979323846;
 
 __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
@@ -90,19 +92,24 @@ struct Lobe {
   float hx, hy, hz;  // hb = l + v (formed in FP64)
   float ux, uy, uz;  // hb x (n x hb) = n |hb|^2 - hb (n.hb) (formed in FP64)
   float rs;    // 1/|l+v|
+  float rs2;   // rs^2
   float ch;    // l.h
+  float sch;   // sat(l.h)
   float nh;    // n.h
   float om4;   // (1 - sat(ch))^4
-  float F, G, D;
+  float F;
+  float Gq;    // G / 4, G = nl ia ib
+  float D;     // alpha^2 / max(pi tD^2, thres)
   float ia;    // 1/(nl(1-k)+k)
-  float it2;   // 1/max(pi tD^2, thres)
   float tD;    // (n.h)^2 (alpha^2-1) + 1
+  float tD2;   // tD^2
+  float itD2;  // 1/max(tD^2, thres/pi) = pi / max(pi tD^2, thres)
   float tmp;   // F G D / 4, or 0 where the reference skips the lobe (tmp <= 0 or NaN)
   bool spec_on;
 };
 
 // S provides nx..nz, vx..vz (fp32), nxd..nzd, vxd..vzd (the same in FP64),
-// k, omk, ib, a2, am1 (per-point constants).
+// k, omk, ib4, a2, a2pi (per-point constants).
 //
 // Geometry of the half vector in FP64 (v_fma_f64 runs at the fp32 vector rate
 // on gfx950): hb = l + v, |hb|^2 and n.hb are formed exactly enough that the
@@ -114,6 +121,8 @@ struct Lobe {
 // holds for any |n|, as the reference's form does.  The backward's
 // n - (n.h) h = (hb x (n x hb)) / |hb|^2 is formed the same way, as
 // n |hb|^2 - hb (n.hb) in FP64 (size ~ sin(theta_h), no 1/sin cancellation).
+// The constant factors 1/4 (of F G D / 4) and 1/pi (of D) ride on the
+// per-point constants ib4 = ib/4 and a2pi = alpha^2/pi.
 template <bool SPEC, class S>
 __device__ __forceinline__ Lobe eval_lobe(const S& s, const Texel& g) {
   Lobe o;
@@ -134,26 +143,29 @@ __device__ __forceinline__ Lobe eval_lobe(const S& s, const Texel& g) {
     const float hb2f = (float)hb2;
     // floor keeps (n.h)^2 finite when v = -l exactly; such pairs are masked below
     o.rs = rsq(fmaxf(hb2f, 1e-20f));
-    const float rs2 = o.rs * o.rs;
+    o.rs2 = o.rs * o.rs;
     o.ch = fmaf(g.lz, o.hz, fmaf(g.ly, o.hy, g.lx * o.hx)) * o.rs;
+    o.sch = sat(o.ch);
     o.nh = (float)nhb * o.rs;
-    const float om = 1.f - sat(o.ch);
+    const float om = 1.f - o.sch;
     const float om2 = om * om;
     o.om4 = om2 * om2;
     o.F = fmaf(o.om4 * om, 1.f - kF0, kF0);
     o.ia = rcp(fmaf(o.nl, s.omk, s.k));
-    o.G = o.nl * o.ia * s.ib;
-    o.tD = fmaf(o.nh * o.nh, s.a2, sn2 * rs2);
-    o.it2 = rcp(fmaxf(kPi * o.tD * o.tD, kThres));
-    o.D = s.a2 * o.it2;
-    const float tmp = o.F * o.G * o.D * 0.25f;
+    o.Gq = o.nl * o.ia * s.ib4;
+    o.tD = fmaf(o.nh * o.nh, s.a2, sn2 * o.rs2);
+    o.tD2 = o.tD * o.tD;
+    o.itD2 = rcp(fmaxf(o.tD2, kThresOverPi));
+    o.D = s.a2pi * o.itD2;
+    const float tmp = o.F * o.Gq * o.D;
     // the reference skips the lobe unless tmp > 0; |l+v| = 0 gives NaN there
     o.spec_on = (tmp > 0.f) && (hb2f > 0.f);
     o.tmp = o.spec_on ? tmp : 0.f;
   } else {
     o.hx = o.hy = o.hz = 0.f;
     o.ux = o.uy = o.uz = 0.f;
-    o.rs = o.ch = o.nh = o.om4 = o.F = o.G = o.D = o.ia = o.it2 = o.tD = 0.f;
+    o.rs = o.rs2 = o.ch = o.sch = o.nh = o.om4 = o.F = o.Gq = o.D = o.ia = 0.f;
+    o.tD = o.tD2 = o.itD2 = 0.f;
     o.spec_on = false;
     o.tmp = 0.f;
   }
@@ -164,7 +176,7 @@ __device__ __forceinline__ Lobe eval_lobe(const S& s, const Texel& g) {
 struct PointGeom {
   float nx, ny, nz, vx, vy, vz;
   double nxd, nyd, nzd, vxd, vyd, vzd;
-  float nv, k, omk, ib, a2, am1;
+  float nv, k, omk, ib4, a2, a2pi, c4;
 };
 
 __device__ __forceinline__ PointGeom point_geom(float nx, float ny, float nz, float vx, float vy,
@@ -178,10 +190,11 @@ __device__ __forceinline__ PointGeom point_geom(float nx, float ny, float nz, fl
   // bs:2126-2129: alpha2 = r^4, k = (r+1)^2 / 8
   const float r2 = r * r;
   c.a2 = r2 * r2;
-  c.am1 = c.a2 - 1.f;
+  c.a2pi = c.a2 * kInvPi;
+  c.c4 = -4.f * (c.a2 - 1.f);  // d tD / d(n.h) = 2 (alpha^2 - 1) n.h, times the -2 of d D / d tD
   c.k = (r + 1.f) * (r + 1.f) * 0.125f;
   c.omk = 1.f - c.k;
-  c.ib = 1.f / (fmaxf(c.nv, 0.f) * c.omk + c.k);
+  c.ib4 = 0.25f / (fmaxf(c.nv, 0.f) * c.omk + c.k);
   return c;
 }
 
