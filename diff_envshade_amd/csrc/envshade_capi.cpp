@@ -90,11 +90,12 @@ des::Tiling choose_tiling(int64_t P, int T, bool backward) {
   if (ov > 0) {
     ntc = std::min(ov, cap);
   } else {
-    // Measured on MI355X from 20k to 262k points: a tile runs best over 8
-    // waves in the forward and 4 in the backward (whose register use allows
-    // 3 waves per SIMD, i.e. three 4-wave tiles per CU).  Problems of fewer
-    // than one wave per SIMD at that split use more waves per tile.
-    const int base = backward ? 4 : 8;
+    // Measured on MI355X from 20k to 262k points with 512-texel super-chunks
+    // (37 KB of LDS per tile): a tile runs best over 4 waves in both
+    // directions -- four or five 4-wave tiles per CU in the forward, three in
+    // the backward (whose register use allows 3 waves per SIMD).  Problems of
+    // fewer than one wave per SIMD at that split use more waves per tile.
+    const int base = 4;
     while (ntc * 2 <= base && ntc * 2 <= nslot) ntc *= 2;
     while (ntc * 2 <= cap && ntc * 2 <= nslot && tiles * ntc < kMinWaves) ntc *= 2;
   }
@@ -322,7 +323,7 @@ des::BatchLaunch batch_launch(void* ws, int variant, bool diff, bool spec, bool 
   bl.hdr_mut = static_cast<des::BatchHeader*>(ws);
   bl.sets = reinterpret_cast<const des::BatchSet*>(static_cast<char*>(ws) + kHeaderBytes);
   bl.ntc = ntc;
-  // table of up to 2048 texels, or the cross-chunk reduction scratch
+  // table of one super-chunk, or the cross-chunk reduction scratch
   const des::Tiling tl = tiling_for(ntc, des::kMaxSuperChunk, !fwd);
   bl.lds = fwd ? tl.lds_fwd : tl.lds_bwd;
   const int per_cu = des::batch_blocks_per_cu(variant, diff, spec, fwd, ntc, bl.lds);

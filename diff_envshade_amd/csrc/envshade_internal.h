@@ -100,7 +100,7 @@ int batch_blocks_per_cu(int variant, bool diff, bool spec, bool fwd, int ntc, si
 
 constexpr int kForwardAcc = 6;
 constexpr int kBackwardAcc = 16;
-constexpr int kMaxSuperChunk = 1024;  // texels staged in LDS at once (48 B each, 48 KB)
+constexpr int kMaxSuperChunk = 512;  // texels staged in LDS at once (48 B each, 24 KB)
 constexpr int kTexelBytes = 48;
 constexpr int kReduceRows = 16;
 
