@@ -52,8 +52,12 @@ constexpr int kTileP = 64;
 constexpr int64_t kMaxTexels = int64_t(1) << 24;
 constexpr int kMinWaves = 1024;  // one wave per SIMD on 256 CUs x 4 SIMDs
 // Automatic point order from this many point-texel pairs up: below it the
-// three sort launches cost about what the skipped texels save.
-constexpr int64_t kOrderMinPairs = int64_t(1) << 24;
+// three sort launches (~15 us) cost about what the skipped texels save.  The
+// forward, at a third of the backward's work per pair, needs larger problems
+// (measured on MI355X: 70k x 512 pairs shade faster unsorted in the forward,
+// 150k x 2048 faster sorted in both directions).
+constexpr int64_t kOrderMinPairsBackward = int64_t(1) << 24;
+constexpr int64_t kOrderMinPairsForward = int64_t(1) << 26;
 
 int64_t n_tiles(int64_t P) { return (P + kTileP - 1) / kTileP; }
 
@@ -105,11 +109,11 @@ des::Tiling choose_tiling(int64_t P, int T, bool backward) {
 size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
 // Whether the single-problem entry points sort the points by normal direction.
-bool use_order(int64_t P, int T) {
+bool use_order(int64_t P, int T, bool backward) {
   if (P <= 0 || P > int64_t(0x7fffffff)) return false;  // the order is int32
   const int mode = g_order_mode.load();
   if (mode != 0) return mode > 0;
-  return P * int64_t(T) >= kOrderMinPairs;
+  return P * int64_t(T) >= (backward ? kOrderMinPairsBackward : kOrderMinPairsForward);
 }
 
 struct WsLayout {
@@ -127,7 +131,7 @@ WsLayout ws_layout(int64_t P, int T, bool allow_order) {
   w.part_off = 0;
   w.part2_off = align256(size_t(tiles) * 3 * T * sizeof(float));
   w.total = w.part2_off + align256(size_t(groups) * 3 * T * sizeof(double));
-  if (allow_order && use_order(P, T)) {
+  if (allow_order && use_order(P, T, true)) {
     w.ordered = true;
     w.order_off = w.total;
     w.total += des::order_layout(P).total;
@@ -171,7 +175,7 @@ int launch_error(hipError_t e, const char* what) {
 int run_forward(int variant, bool diff, bool spec, des::Problem& pr, void* ws, size_t ws_bytes,
                 hipStream_t stream) {
   if (pr.P == 0) return ok();
-  if (ws != nullptr && use_order(pr.P, pr.T)) {
+  if (ws != nullptr && use_order(pr.P, pr.T, false)) {
     const des::OrderLayout ol = des::order_layout(pr.P);
     if (ws_bytes < ol.total)
       return fail(DES_ERR_WORKSPACE_TOO_SMALL, "workspace of %zu bytes is smaller than the %zu required",
@@ -346,7 +350,7 @@ DES_API size_t des_backward_workspace_size(int64_t P, int32_t H, int32_t W) {
 }
 
 DES_API size_t des_forward_workspace_size(int64_t P, int32_t H, int32_t W) {
-  if (P <= 0 || H <= 0 || W <= 0 || !use_order(P, H * W)) return 0;
+  if (P <= 0 || H <= 0 || W <= 0 || !use_order(P, H * W, false)) return 0;
   return des::order_layout(P).total;
 }
 

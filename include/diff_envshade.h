@@ -78,7 +78,8 @@ DES_API size_t des_backward_workspace_size(int64_t P, int32_t H, int32_t W);
 DES_API size_t des_forward_workspace_size(int64_t P, int32_t H, int32_t W);
 
 /* Point-order policy (no counterpart in the reference), process-wide:
- * 0 = automatic (reorder when P x H x W is large), 1 = always, -1 = never.
+ * 0 = automatic (the backward reorders from 2^24 point-texel pairs P x H x W,
+ * the forward from 2^26), 1 = always, -1 = never.
  * The workspace sizes above follow the policy in force when they are asked. */
 DES_API int des_set_point_order(int32_t mode);
 

@@ -80,10 +80,13 @@ def test_forward_workspace_follows_point_order_policy():
         _lib.set_point_order(1)
         fw = _lib.forward_workspace_size(100, 4, 8)
         assert fw >= 100 * 4 and fw % 256 == 0
-        assert _lib.workspace_size(70000, 16, 32) == never + _lib.forward_workspace_size(70000, 16, 32)
+        order = _lib.forward_workspace_size(70000, 16, 32)
+        assert _lib.workspace_size(70000, 16, 32) == never + order
         _lib.set_point_order(0)
-        # automatic: the headline problem (35.8 M pairs) is reordered, a tiny one is not
-        assert _lib.forward_workspace_size(70000, 16, 32) > 0
+        # automatic: the backward reorders from 2^24 pairs, the forward from 2^26
+        assert _lib.forward_workspace_size(70000, 16, 32) == 0  # 35.8 M pairs
+        assert _lib.workspace_size(70000, 16, 32) == never + order
+        assert _lib.forward_workspace_size(40000, 32, 64) > 0  # 81.9 M pairs
         assert _lib.forward_workspace_size(100, 4, 8) == 0
         with pytest.raises(RuntimeError):
             _lib.set_point_order(2)
