@@ -690,12 +690,13 @@ __global__ __launch_bounds__(512) void batch_forward_kernel(const BatchHeader* _
     if (g >= total) break;
     const int s = __builtin_amdgcn_readfirstlane(
         find_set(n, g, [&](int i) { return sets[i].tile0; }));
-    forward_tile<V, DIFF, SPEC, false>(sets[s].pr, sets[s].tl_fwd, g - sets[s].tile0);
+    forward_tile<V, DIFF, SPEC, true>(sets[s].pr, sets[s].tl_fwd, g - sets[s].tile0);
   }
 }
 
 template <int V, bool DIFF, bool SPEC>
-__global__ __launch_bounds__(512) void batch_backward_kernel(const BatchHeader* __restrict__ hdr,
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(3))) void batch_backward_kernel(
+    const BatchHeader* __restrict__ hdr,
                                                              const BatchSet* __restrict__ sets,
                                                              unsigned long long* counter) {
   if (!hdr->has_backward) return;
@@ -706,7 +707,7 @@ __global__ __launch_bounds__(512) void batch_backward_kernel(const BatchHeader* 
     if (g >= total) break;
     const int s = __builtin_amdgcn_readfirstlane(
         find_set(n, g, [&](int i) { return sets[i].tile0; }));
-    backward_tile<V, DIFF, SPEC, false>(sets[s].pr, sets[s].tl_bwd, g - sets[s].tile0);
+    backward_tile<V, DIFF, SPEC, true>(sets[s].pr, sets[s].tl_bwd, g - sets[s].tile0);
   }
 }
 
