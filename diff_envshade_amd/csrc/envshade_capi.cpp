@@ -96,9 +96,9 @@ des::Tiling choose_tiling(int64_t P, int T, bool backward) {
   } else {
     // Measured on MI355X from 20k to 262k points with 512-texel super-chunks
     // (37 KB of LDS per tile): a tile runs best over 4 waves in both
-    // directions -- four or five 4-wave tiles per CU in the forward, three in
-    // the backward (whose register use allows 3 waves per SIMD).  Problems of
-    // fewer than one wave per SIMD at that split use more waves per tile.
+    // directions, four 4-wave tiles per CU (the backward is held to 128
+    // VGPRs, 4 waves per SIMD).  Problems of fewer than one wave per SIMD at
+    // that split use more waves per tile.
     const int base = 4;
     while (ntc * 2 <= base && ntc * 2 <= nslot) ntc *= 2;
     while (ntc * 2 <= cap && ntc * 2 <= nslot && tiles * ntc < kMinWaves) ntc *= 2;

@@ -49,7 +49,7 @@ struct Problem {
 // Launch shape: one 64-point tile per workgroup, NTC waves over texel chunks.
 struct Tiling {
   int ntc;          // waves (texel chunks) per workgroup
-  int cw;           // texels per wave per super-chunk (multiple of 32)
+  int cw;           // super-chunk texels per wave, st / ntc (multiple of 32)
   int st;           // texels per super-chunk = ntc * cw
   int vis_off;      // byte offset of the staged visibility words in dynamic LDS
   size_t lds_fwd;   // dynamic LDS bytes, forward

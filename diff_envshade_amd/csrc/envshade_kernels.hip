@@ -160,17 +160,6 @@ __device__ __forceinline__ float vis_weight(const PointState<V, SPEC>& s, const 
   return fmaf(b2, s.c2, fmaf(b1, s.c1, b0 * s.c0));
 }
 
-// Texel groups: a super-chunk of st = ntc * cw texels is cut into groups of 4;
-// wave wv takes group k * ntc + ((wv + k) mod ntc) for k = 0 .. cw/4 - 1, a
-// diagonal across the rows of the lat-long map.  Each wave thus sees every
-// latitude and longitude band, so the waves of a tile get about the same share
-// of the texels that light none of its points (which are skipped), whatever
-// direction the tile's normals face.  Returns the group's first texel within
-// the super-chunk (a multiple of 4; a group never straddles a 32-texel slot).
-__device__ __forceinline__ int group_texel(int k, int wv, int ntc) {
-  return 4 * (k * ntc + ((wv + k) & (ntc - 1)));
-}
-
 // The tile's visibility words for the slots of the super-chunk at s0 (sn
 // texels) are staged in LDS once per super-chunk -- word of vertex j of lane l
 // for local slot sl at vis[(sl * 3 + j) * 64 + l] -- so that the texel loop
@@ -264,40 +253,81 @@ __device__ __forceinline__ Cone tile_cone(const PointState<V, SPEC>& s) {
   return c;
 }
 
-// Dark bits of the 16 texel groups k0 .. k0+15 of wave wv: bit 4 i + j is
-// texel j of group k0 + i (groups past ng read as dark).
-__device__ __forceinline__ uint64_t dark_mask(const Texel* tab, const Cone& c, int k0, int ng,
-                                              int wv, int ntc, int lane) {
-  const int k = k0 + (lane >> 2);
+// Dark bits of texels t0 .. t0+63 of the staged super-chunk (sn texels), bit
+// i for texel t0 + i; texels past sn read as dark.  Wave-uniform.
+__device__ __forceinline__ uint64_t dark_word(const Texel* tab, const Cone& c, int t0, int sn,
+                                              int lane) {
+  const int t = t0 + lane;
   bool dark = true;
-  if (k < ng) {
-    const Texel& g = tab[group_texel(k, wv, ntc) + (lane & 3)];
+  if (t < sn) {
+    const Texel& g = tab[t];
     const float len = __builtin_amdgcn_sqrtf(fmaf(g.lz, g.lz, fmaf(g.ly, g.ly, g.lx * g.lx)));
     dark = fmaf(g.lz, c.az, fmaf(g.ly, c.ay, g.lx * c.ax)) <= -c.sb * len;
   }
   return __ballot(dark);
 }
 
+// Texel groups.  A staged super-chunk of sn texels is cut into groups of 4
+// (a group never straddles a 32-texel visibility slot).  Every wave of the
+// tile walks the groups in order: the lit groups -- those that light some
+// lane -- are dealt to the ntc waves in turn, so each wave shades the same
+// number of them (+-1) whatever way the tile faces, and a dark group k belongs
+// to wave k mod ntc, which only writes its zero env-map partials.  The deal
+// depends only on the tile's dark bits, so the sums are reproducible.
+// lit(tg, d4) / dark(tg) get the group's first texel and, for lit groups, its
+// 4 dark bits.
+template <class Lit, class Dark>
+__device__ __forceinline__ void for_groups(const Texel* tab, const Cone& cone, int sn, int wv,
+                                           int ntc, int lane, Lit&& lit, Dark&& dark) {
+  const int ngt = (sn + 3) >> 2;
+  // a cone wider than a hemisphere darkens no texel: skip the dark bits
+  const bool wide = cone.sb >= 1.f;
+  int c = 0;  // lit groups before group k0 (wave-uniform)
+#pragma unroll 1
+  for (int k0 = 0; k0 < ngt; k0 += 16) {
+    const int kn = min(16, ngt - k0);
+    // Which of these 16 groups this wave owns, one group per lane (lane i
+    // looks at group k0 + i), before any shading: the shading loop below then
+    // has one exit, and its accumulators need no copies at control-flow joins.
+    const uint64_t dk = wide ? 0ull : dark_word(tab, cone, 4 * k0, sn, lane);
+    const uint64_t all4 = dk & (dk >> 1) & (dk >> 2) & (dk >> 3);  // bit 4 i: group i dark
+    const bool in = lane < kn;
+    const bool gdark = in && ((all4 >> (4 * (lane & 15))) & 1ull) != 0ull;
+    const bool glit = in && !gdark;
+    const uint64_t litm = __ballot(glit);
+    const int rank = c + (int)__builtin_popcountll(litm & ((1ull << lane) - 1ull));
+    uint32_t own = (uint32_t)__ballot(glit && (rank & (ntc - 1)) == wv);
+    uint32_t own_dark = (uint32_t)__ballot(gdark && ((k0 + lane) & (ntc - 1)) == wv);
+    c += (int)__builtin_popcountll(litm);
+#pragma unroll 1
+    while (own_dark != 0u) {
+      const int i = __builtin_ctz(own_dark);
+      own_dark &= own_dark - 1u;
+      dark(4 * (k0 + i));
+    }
+#pragma unroll 1
+    while (own != 0u) {
+      const int i = __builtin_ctz(own);
+      own &= own - 1u;
+      lit(4 * (k0 + i), (uint32_t)(dk >> (4 * i)) & 0xfu);
+    }
+  }
+}
+
 // ---- forward -----------------------------------------------------------------
 
-// One wave's texel groups of one staged super-chunk (sn texels, ng groups).
+// One wave's share of the texel groups of one staged super-chunk (sn texels).
 // A group that lights some lane is shaded whole, without a branch per texel,
 // so the four lobes form one block the compiler can interleave; its dark
 // texels add exact zeros (see Cone).  SINGLE: the wave-uniform case of one
 // visibility word per slot, chosen once per super-chunk.
 // acc = {sum E vis max(l.n,0) per channel, sum E vis F G D / 4 per channel}.
-template <int V, bool DIFF, bool SPEC, bool CULL, bool SINGLE>
+template <int V, bool DIFF, bool SPEC, bool SINGLE>
 __device__ __forceinline__ void forward_chunk(const PointState<V, SPEC>& st, const Texel* tab,
                                               const uint32_t* svis, const Cone& cone, int sn,
-                                              int ng, int wv, int ntc, int lane, float csum,
+                                              int wv, int ntc, int lane, float csum,
                                               float (&acc)[6]) {
-  uint64_t dark = 0ull;
-  for (int k = 0; k < ng; ++k) {
-    const int tg = group_texel(k, wv, ntc);
-    if (tg >= sn) break;
-    if (CULL && (k & 15) == 0) dark = dark_mask(tab, cone, k, ng, wv, ntc, lane);
-    const uint32_t d4 = CULL ? (uint32_t)(dark >> (4 * (k & 15))) & 0xfu : 0u;
-    if (d4 == 0xfu) continue;  // lights no lane of the tile: adds exactly 0
+  for_groups(tab, cone, sn, wv, ntc, lane, [&](int tg, uint32_t) {
     const VisWords w = staged_words<V>(svis, tg >> 5, SINGLE, lane);
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
@@ -317,14 +347,14 @@ __device__ __forceinline__ void forward_chunk(const PointState<V, SPEC>& st, con
         acc[5] = fmaf(g.e2, y, acc[5]);
       }
     }
-  }
+  }, [](int) {});
 }
 
 // One 64-point tile of one problem; called by every wave of the workgroup.
 // Waves other than wave 0 return after the cross-chunk reduction; wave 0
 // writes the outputs.  No barrier follows the last one executed here, so a
 // caller reusing LDS must synchronise first.
-template <int V, bool DIFF, bool SPEC, bool CULL>
+template <int V, bool DIFF, bool SPEC>
 __device__ __forceinline__ void forward_tile(const Problem& pr, const Tiling& tl, int64_t tile) {
   extern __shared__ float4 smem[];
   constexpr bool DOUT = (V == kV2 || V == kVis);
@@ -343,8 +373,7 @@ __device__ __forceinline__ void forward_tile(const Problem& pr, const Tiling& tl
     csum = st.c0 + st.c1 + st.c2;
   }
 
-  Cone cone{0.f, 0.f, 0.f, 4.f};
-  if (CULL) cone = tile_cone(st);
+  const Cone cone = tile_cone(st);
 
   constexpr int NACC = kForwardAcc;
   float acc[NACC] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
@@ -357,11 +386,10 @@ __device__ __forceinline__ void forward_tile(const Problem& pr, const Tiling& tl
     build_table(pr, s0, sn, tl.st, tab);
     stage_vis(pr, st, s0, sn, single, svis, wv, tl.ntc, lane);
     __syncthreads();
-    const int ng = tl.cw >> 2;
     if (single)
-      forward_chunk<V, DIFF, SPEC, CULL, true>(st, tab, svis, cone, sn, ng, wv, tl.ntc, lane, csum, acc);
+      forward_chunk<V, DIFF, SPEC, true>(st, tab, svis, cone, sn, wv, tl.ntc, lane, csum, acc);
     else
-      forward_chunk<V, DIFF, SPEC, CULL, false>(st, tab, svis, cone, sn, ng, wv, tl.ntc, lane, csum, acc);
+      forward_chunk<V, DIFF, SPEC, false>(st, tab, svis, cone, sn, wv, tl.ntc, lane, csum, acc);
   }
 
   // fixed-order reduction over the NTC texel chunks
@@ -408,31 +436,20 @@ struct BwdSums {
   float gk;             // sum grad_a nl (nl - 1) ia^2
 };
 
-// One wave's texel groups of one staged super-chunk at s0 (sn texels, ng
-// groups): per-point sums into a, and per group the tile's env-map partials of
-// its 4 texels into part.  Unlike the forward, dark texels of a lit group are
-// skipped one by one: four backward lobes in one block need more than the
+// One wave's share of the texel groups of one staged super-chunk at s0 (sn
+// texels): per-point sums into a, and per group the tile's env-map partials
+// of its 4 texels into part.  Unlike the forward, dark texels of a lit group
+// are skipped one by one: four backward lobes in one block need more than the
 // 168 registers that keep three waves per SIMD.
-template <int V, bool DIFF, bool SPEC, bool CULL>
+template <int V, bool DIFF, bool SPEC>
 __device__ __forceinline__ void backward_chunk(const PointState<V, SPEC>& st, const Texel* tab,
                                                const uint32_t* svis, const Cone& cone, int s0,
-                                               int sn, int ng, int wv, int ntc, int lane,
-                                               bool single, float csum, const float* A,
-                                               const float* B, int T, float* part, int row_texel,
-                                               int ch_out, BwdSums& a) {
-  uint64_t dark = 0ull;
-#pragma unroll 1
-  for (int k = 0; k < ng; ++k) {
-    const int tg = group_texel(k, wv, ntc);
-    if (tg >= sn) break;
-    if (CULL && (k & 15) == 0) dark = dark_mask(tab, cone, k, ng, wv, ntc, lane);
-    const uint32_t d4 = CULL ? (uint32_t)(dark >> (4 * (k & 15))) & 0xfu : 0u;
+                                               int sn, int wv, int ntc, int lane, bool single,
+                                               float csum, const float* A, const float* B, int T,
+                                               float* part, int row_texel, int ch_out,
+                                               BwdSums& a) {
+  for_groups(tab, cone, sn, wv, ntc, lane, [&](int tg, uint32_t d4) {
     const int t_out = s0 + tg + row_texel;
-    if (d4 == 0xfu) {
-      // the 4 texels light no lane of the tile: their partials are exactly 0
-      if (ch_out < 3 && t_out < T) part[(int64_t)ch_out * T + t_out] = 0.f;
-      continue;
-    }
     const VisWords w = staged_words<V>(svis, tg >> 5, single, lane);
     const int b4 = tg & 31;
     {
@@ -518,7 +535,11 @@ __device__ __forceinline__ void backward_chunk(const PointState<V, SPEC>& st, co
         part[(int64_t)ch_out * T + t_out] = val;
       }
     }
-  }
+  }, [&](int tg) {
+    // the 4 texels light no lane of the tile: their partials are exactly 0
+    const int t_out = s0 + tg + row_texel;
+    if (ch_out < 3 && t_out < T) part[(int64_t)ch_out * T + t_out] = 0.f;
+  });
 }
 
 // ---- backward ----------------------------------------------------------------
@@ -533,7 +554,7 @@ __device__ __forceinline__ void backward_chunk(const PointState<V, SPEC>& st, co
 //   terms of the NDF and normalize VJPs are accumulated along l + v and along
 //   hb x (n x hb); the G1 parts along n and v are per-point constants times a
 //   scalar sum (Sb), applied once at the end.
-template <int V, bool DIFF, bool SPEC, bool CULL>
+template <int V, bool DIFF, bool SPEC>
 __device__ __forceinline__ void backward_tile(const Problem& pr, const Tiling& tl, int64_t tile) {
   extern __shared__ float4 smem[];
   constexpr bool DOUT = (V == kV2 || V == kVis);
@@ -565,8 +586,7 @@ __device__ __forceinline__ void backward_tile(const Problem& pr, const Tiling& t
     }
   }
 
-  Cone cone{0.f, 0.f, 0.f, 4.f};
-  if (CULL) cone = tile_cone(st);
+  const Cone cone = tile_cone(st);
 
   BwdSums a{};
 
@@ -584,9 +604,8 @@ __device__ __forceinline__ void backward_tile(const Problem& pr, const Tiling& t
     build_table(pr, s0, sn, tl.st, tab);
     stage_vis(pr, st, s0, sn, single, svis, wv, tl.ntc, lane);
     __syncthreads();
-    const int ng = tl.cw >> 2;
-    backward_chunk<V, DIFF, SPEC, CULL>(st, tab, svis, cone, s0, sn, ng, wv, tl.ntc, lane, single,
-                                        csum, A, B, pr.T, part, row_texel, ch_out, a);
+    backward_chunk<V, DIFF, SPEC>(st, tab, svis, cone, s0, sn, wv, tl.ntc, lane, single, csum, A,
+                                  B, pr.T, part, row_texel, ch_out, a);
   }
 
   // fixed-order reduction of the per-point sums over the NTC texel chunks
@@ -643,14 +662,14 @@ __device__ __forceinline__ void backward_tile(const Problem& pr, const Tiling& t
 
 template <int V, bool DIFF, bool SPEC>
 __global__ __launch_bounds__(1024) void shade_forward_kernel(Problem pr, Tiling tl) {
-  forward_tile<V, DIFF, SPEC, true>(pr, tl, blockIdx.x);
+  forward_tile<V, DIFF, SPEC>(pr, tl, blockIdx.x);
 }
 
 // Three waves per SIMD (<= 168 VGPRs): the measured best for this kernel.
 template <int V, bool DIFF, bool SPEC>
-__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(3))) void shade_backward_kernel(
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void shade_backward_kernel(
     Problem pr, Tiling tl) {
-  backward_tile<V, DIFF, SPEC, true>(pr, tl, blockIdx.x);
+  backward_tile<V, DIFF, SPEC>(pr, tl, blockIdx.x);
 }
 
 // ---- batched launches: persistent workgroups pull tiles of many problems ------
@@ -690,7 +709,7 @@ __global__ __launch_bounds__(512) void batch_forward_kernel(const BatchHeader* _
     if (g >= total) break;
     const int s = __builtin_amdgcn_readfirstlane(
         find_set(n, g, [&](int i) { return sets[i].tile0; }));
-    forward_tile<V, DIFF, SPEC, true>(sets[s].pr, sets[s].tl_fwd, g - sets[s].tile0);
+    forward_tile<V, DIFF, SPEC>(sets[s].pr, sets[s].tl_fwd, g - sets[s].tile0);
   }
 }
 
@@ -707,7 +726,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(3))) void b
     if (g >= total) break;
     const int s = __builtin_amdgcn_readfirstlane(
         find_set(n, g, [&](int i) { return sets[i].tile0; }));
-    backward_tile<V, DIFF, SPEC, true>(sets[s].pr, sets[s].tl_bwd, g - sets[s].tile0);
+    backward_tile<V, DIFF, SPEC>(sets[s].pr, sets[s].tl_bwd, g - sets[s].tile0);
   }
 }
 

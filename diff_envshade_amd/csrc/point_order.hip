@@ -11,10 +11,9 @@
 // map, so consecutive keys are neighbouring directions.  The order is a
 // stable counting sort by key -- points with equal keys keep their index
 // order -- computed without atomics on positions, so it depends only on the
-// normals, and the per-tile env-map-gradient partials (whose summation order
-// follows tile membership) are bitwise reproducible.  Per-point outputs do not
-// depend on the order at all: each point's texel sums run in the same order
-// whatever tile it lands in.
+// normals, and every output -- the per-point sums, grouped by the tile's
+// waves, and the per-tile env-map-gradient partials -- is bitwise
+// reproducible; against input order the outputs move at fp32 rounding level.
 //
 // Blocks of 1024 points, one per thread (small blocks spread the ranking over
 // many CUs):

@@ -28,9 +28,10 @@
  *   - point order: for large problems the entry points first sort the points
  *     by normal direction (a stable counting sort, deterministic), so that a
  *     64-point tile shares a hemisphere and texels lit by none of its points
- *     are skipped.  Per-point outputs do not depend on the order; grad_envmap
- *     is summed per tile, so it changes with the order at fp32 rounding level
- *     and is still bitwise reproducible.  The forward entry points take an
+ *     are skipped, and the texel groups a tile lights are dealt evenly to its
+ *     waves.  Outputs change with the order at fp32 rounding level only (each
+ *     sum is regrouped) and stay bitwise reproducible run to run: the order and
+ *     the deal depend only on the inputs.  The forward entry points take an
  *     optional workspace of des_forward_workspace_size(P, H, W) bytes for the
  *     order (NULL = shade in input order); the backward workspace includes it;
  *   - return 0 (DES_OK) or a negative DES_ERR_* code; des_last_error() gives

@@ -1,13 +1,14 @@
 """Point order by normal direction (GPU): sorting the points into
-normal-coherent tiles and skipping texels that light no lane of a tile must
-not change any per-point output (bitwise, up to the sign of zero), and may
-change grad_envmap only at fp32 rounding level while staying bitwise
-reproducible.  Oracle parity of the sorted path is covered by the parity
-suite, which runs every case with the order forced on."""
+normal-coherent tiles, skipping texels that light no lane of a tile and
+dealing the lit texel groups evenly to the tile's waves may change the
+outputs only at fp32 rounding level (every sum is regrouped), and the
+results stay bitwise reproducible.  Oracle parity of the sorted path is
+covered by the parity suite, which runs every case with the order forced
+on."""
 import numpy as np
 import pytest
 
-from _helpers import compare, gpu_run, oracle_run, problem, relerr
+from _helpers import TOL, compare, gpu_run, oracle_run, problem, relerr
 
 pytestmark = pytest.mark.gpu
 
@@ -24,19 +25,22 @@ def _run(mode, variant, d, diff=True, spec=True):
 @pytest.mark.parametrize("variant,diff,spec", [("vis", True, True), ("v2", True, True),
                                                ("clamp", True, True), ("specular", False, True),
                                                ("diffuse", True, False)])
-def test_per_point_outputs_do_not_depend_on_the_order(variant, diff, spec):
+def test_point_order_changes_outputs_only_by_rounding(variant, diff, spec):
+    # every sum is regrouped; sums that cancel (grad_roughness) move by up to
+    # ~2e-6 normwise, so the two orders are held to the 1e-5 accuracy bar they
+    # each meet against FP64 in the parity suite
     d = problem(seed=21, n=20000, H=16, W=32, vis="identity" if variant == "vis" else "none")
     (f0, b0), (f1, b1) = _run(-1, variant, d, diff, spec), _run(1, variant, d, diff, spec)
-    for a, b in zip(f0 + b0[:-1], f1 + b1[:-1]):
-        assert np.array_equal(a, b)  # -0.0 == 0.0
-    assert relerr(b1[-1], b0[-1]) <= 1e-6
+    for a, b in zip(f0 + b0, f1 + b1):
+        assert relerr(b, a) <= TOL
 
 
-def test_sorted_grad_envmap_is_bitwise_reproducible():
+def test_sorted_outputs_are_bitwise_reproducible():
     d = problem(seed=22, n=30000, H=16, W=32, vis="identity")
-    a = _run(1, "vis", d)[1][-1]
-    b = _run(1, "vis", d)[1][-1]
-    assert np.array_equal(a, b)
+    f0, b0 = _run(1, "vis", d)
+    f1, b1 = _run(1, "vis", d)
+    for a, b in zip(f0 + b0, f1 + b1):
+        assert np.array_equal(a, b)
 
 
 def test_coherent_normals_skip_half_the_map():
@@ -68,6 +72,5 @@ def test_order_block_boundaries(n):
     # points: 65 blocks, past the fused limit (separate per-key scan)
     d = problem(seed=25, n=n, H=8, W=16, vis="identity")
     (f0, b0), (f1, b1) = _run(-1, "vis", d), _run(1, "vis", d)
-    for a, b in zip(f0 + b0[:-1], f1 + b1[:-1]):
-        assert np.array_equal(a, b)
-    assert relerr(b1[-1], b0[-1]) <= 1e-6
+    for a, b in zip(f0 + b0, f1 + b1):
+        assert relerr(b, a) <= TOL
