@@ -61,6 +61,8 @@ def _bind(L):
     L.des_batch_forward.restype = c_int
     L.des_batch_backward.argtypes = [i32, c_int, c_int, i32, _vp, _vp]
     L.des_batch_backward.restype = c_int
+    L.des_batch_sort.argtypes = [_vp, _vp]
+    L.des_batch_sort.restype = c_int
     L._batch_bound = True
     return L
 
@@ -176,14 +178,21 @@ class ShadeBatch:
                                              int(self.enable_specular), self.ntc_backward,
                                              self.workspace.data_ptr(), s))
 
+    def sort(self):
+        """Recompute the point order of the prepared sets from the normals'
+        current values (prepare() computes it once; des_batch_sort)."""
+        s = torch.cuda.current_stream(self.device).cuda_stream
+        _lib.check(self.L.des_batch_sort(self.workspace.data_ptr(), s))
+
     def capture(self, with_backward: bool = True):
-        """Capture forward (+ backward) into a HIP graph; replay() reruns it
-        for whatever batch was last prepared into this workspace."""
+        """Capture sort + forward (+ backward) into a HIP graph; replay() reruns
+        it for whatever batch was last prepared into this workspace."""
         if self.workspace is None:
             raise RuntimeError("prepare() a batch before capturing")
         torch.cuda.synchronize(self.device)
         g = torch.cuda.CUDAGraph()
         with torch.cuda.graph(g):
+            self.sort()
             self.forward()
             if with_backward:
                 self.backward()

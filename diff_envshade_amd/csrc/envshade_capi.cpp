@@ -8,6 +8,7 @@
 #include <atomic>
 #include <cstdarg>
 #include <cstdio>
+#include <cstddef>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -247,23 +248,39 @@ des::Problem make_problem(int64_t P, int32_t H, int32_t W) {
 constexpr size_t kHeaderBytes = 256;
 static_assert(sizeof(des::BatchHeader) <= kHeaderBytes, "header too large");
 
+// Batch workspace: the descriptor table (header, one BatchSet per set, one
+// SortJob per sorted set), then per set its env-gradient partials and, when
+// the set is sorted, its point-order workspace.  A set is sorted when either
+// direction's policy asks for it (the backward's threshold is the lower one);
+// the order is computed once, at prepare time, for both directions.
 struct BatchLayout {
-  size_t sets_off, table_bytes, total;
-  std::vector<size_t> part_off, part2_off;
+  size_t sets_off, jobs_off, table_bytes, total;
+  int njobs;
+  std::vector<size_t> part_off, part2_off, order_off;  // order_off: 0 = not sorted
 };
 
 BatchLayout batch_layout(const des_set* sets, int n) {
   BatchLayout L;
   L.sets_off = kHeaderBytes;
-  L.table_bytes = align256(kHeaderBytes + sizeof(des::BatchSet) * size_t(n));
+  L.jobs_off = (kHeaderBytes + sizeof(des::BatchSet) * size_t(n) + 15) & ~size_t(15);
+  L.njobs = 0;
+  for (int i = 0; i < n; ++i)
+    if (use_order(sets[i].P, sets[i].H * sets[i].W, true)) ++L.njobs;
+  L.table_bytes = align256(L.jobs_off + offsetof(des::SortJobs, jobs) +
+                           sizeof(des::SortJob) * size_t(L.njobs));
   size_t off = L.table_bytes;
   L.part_off.resize(n);
   L.part2_off.resize(n);
+  L.order_off.assign(n, 0);
   for (int i = 0; i < n; ++i) {
     const WsLayout w = ws_layout(sets[i].P, sets[i].H * sets[i].W, false);
     L.part_off[i] = off + w.part_off;
     L.part2_off[i] = off + w.part2_off;
     off += w.total;
+    if (use_order(sets[i].P, sets[i].H * sets[i].W, true)) {
+      L.order_off[i] = off;
+      off += align256(des::order_layout(sets[i].P).total);
+    }
   }
   L.total = off;
   return L;
@@ -636,6 +653,9 @@ DES_API int des_batch_prepare(int32_t variant, int enable_diffuse, int enable_sp
   auto* hdr = reinterpret_cast<des::BatchHeader*>(st);
   auto* bs = reinterpret_cast<des::BatchSet*>(st + kHeaderBytes);
   char* ws = static_cast<char*>(workspace);
+  auto* tab = reinterpret_cast<des::SortJobs*>(st + L.jobs_off);
+  des::SortJob* jobs = tab->jobs;
+  int njobs = 0, sort_blocks = 0, sort_scan = 0;
   int64_t tile0 = 0, r1 = 0, r2 = 0;
   for (int i = 0; i < n_sets; ++i) {
     const des_set& s = sets[i];
@@ -654,6 +674,16 @@ DES_API int des_batch_prepare(int32_t variant, int enable_diffuse, int enable_sp
     pr.part2 = reinterpret_cast<double*>(ws + L.part2_off[i]);
     des::BatchSet& b = bs[i];
     b.pr = pr;
+    if (L.order_off[i] != 0) {
+      des::SortJob job = des::sort_job(s.normal, s.P, ws + L.order_off[i]);
+      job.blk0 = sort_blocks;
+      job.scan0 = sort_scan;
+      sort_blocks += job.nblk;
+      sort_scan += des::sort_scan_groups(job);
+      jobs[njobs++] = job;
+      if (use_order(s.P, pr.T, false)) b.order_fwd = job.order;
+      b.order_bwd = job.order;
+    }
     b.tl_fwd = tiling_for(ntc_forward, pr.T, false);
     b.tl_bwd = tiling_for(ntc_backward, pr.T, true);
     const int64_t tiles = n_tiles(s.P);
@@ -673,9 +703,25 @@ DES_API int des_batch_prepare(int32_t variant, int enable_diffuse, int enable_sp
   hdr->total_tiles = tile0;
   hdr->total_r1 = r1;
   hdr->total_r2 = r2;
+  hdr->sort_jobs = reinterpret_cast<const des::SortJobs*>(ws + L.jobs_off);
+  tab->njobs = njobs;
+  tab->nblocks = sort_blocks;
+  tab->nscan = sort_scan;
   hipError_t e = hipMemcpyAsync(workspace, staging, L.table_bytes, hipMemcpyHostToDevice,
                                 static_cast<hipStream_t>(stream));
   if (e != hipSuccess) return launch_error(e, "descriptor upload failed");
+  e = des::launch_point_order_jobs(&reinterpret_cast<des::BatchHeader*>(ws)->sort_jobs, num_cus(),
+                                   static_cast<hipStream_t>(stream));
+  if (e != hipSuccess) return launch_error(e, "point order failed");
+  return ok();
+}
+
+DES_API int des_batch_sort(void* workspace, void* stream) {
+  REQUIRE(workspace);
+  hipError_t e = des::launch_point_order_jobs(
+      &static_cast<des::BatchHeader*>(workspace)->sort_jobs, num_cus(),
+      static_cast<hipStream_t>(stream));
+  if (e != hipSuccess) return launch_error(e, "point order failed");
   return ok();
 }
 

@@ -57,9 +57,13 @@ struct Tiling {
   int reduce_rows;  // partial rows summed per stage-1 group
 };
 
+struct SortJobs;  // the point-order jobs of a batch (below)
+
 // Batched launches: a device-resident descriptor table.
 struct BatchSet {
   Problem pr;         // pointers + sizes of one point set (part/part2 inside the workspace)
+  const int32_t* order_fwd;  // point order the forward shades in, or null (input order)
+  const int32_t* order_bwd;  // the same for the backward
   Tiling tl_fwd;      // ntc equals the forward launch's ntc; cw/st follow this set's T
   Tiling tl_bwd;      // same for the backward launch
   int64_t tile0;      // first global tile of this set
@@ -80,6 +84,7 @@ struct BatchHeader {
   int64_t total_r2;
   unsigned long long fwd_counter;  // work counters, zeroed by a kernel at each launch
   unsigned long long bwd_counter;
+  const SortJobs* sort_jobs;       // the sets' point-order jobs (inside the table)
 };
 
 struct BatchLaunch {
@@ -123,5 +128,34 @@ struct OrderLayout {
 };
 OrderLayout order_layout(int64_t P);
 hipError_t launch_point_order(const float* normal, int64_t P, void* ws, hipStream_t stream);
+
+// The sort of one point set as a plain descriptor, so that one launch can sort
+// many sets (a batch): its rank and scatter workgroups are blk0 .. blk0+nblk-1
+// of the launch, its scan workgroups scan0 .. scan0+sort_scan_groups-1.
+struct SortJob {
+  const float* normal;
+  int64_t P;
+  uint32_t* keyrank;
+  uint32_t* count;
+  uint32_t* total;
+  int32_t* order;
+  int32_t nblk, nblk_pad, fused;
+  int32_t blk0, scan0;
+};
+SortJob sort_job(const float* normal, int64_t P, void* ws);  // ws: an order_layout(P) workspace
+int sort_scan_groups(const SortJob& job);
+
+// A device-resident table of sort jobs: njobs entries, nblocks rank/scatter
+// items and nscan scan items in total (jobs[] follows the header).
+struct SortJobs {
+  int32_t njobs, nblocks, nscan, reserved;
+  SortJob jobs[1];
+};
+// Sorts every job of the table *tabp (a device pointer to a device pointer,
+// read when the kernels run) with fixed grids of persistent workgroups
+// (max_groups for rank and scatter, 4x that for the scan); an empty table
+// costs three near-empty launches.
+hipError_t launch_point_order_jobs(const SortJobs* const* tabp, int max_groups,
+                                   hipStream_t stream);
 
 }  // namespace des

@@ -354,15 +354,17 @@ __device__ __forceinline__ void forward_chunk(const PointState<V, SPEC>& st, con
 // Waves other than wave 0 return after the cross-chunk reduction; wave 0
 // writes the outputs.  No barrier follows the last one executed here, so a
 // caller reusing LDS must synchronise first.
+// order: point of each tile position (tile * 64 + lane), or null for input order.
 template <int V, bool DIFF, bool SPEC>
-__device__ __forceinline__ void forward_tile(const Problem& pr, const Tiling& tl, int64_t tile) {
+__device__ __forceinline__ void forward_tile(const Problem& pr, const Tiling& tl, int64_t tile,
+                                             const int32_t* order) {
   extern __shared__ float4 smem[];
   constexpr bool DOUT = (V == kV2 || V == kVis);
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
   const int64_t q = tile * 64 + lane;
   const bool valid = q < pr.P;
-  const int64_t p = (pr.order != nullptr && valid) ? (int64_t)pr.order[q] : q;
+  const int64_t p = (order != nullptr && valid) ? (int64_t)order[q] : q;
 
   PointState<V, SPEC> st;
   load_point<V, SPEC>(pr, p, valid, st);
@@ -555,14 +557,15 @@ __device__ __forceinline__ void backward_chunk(const PointState<V, SPEC>& st, co
 //   hb x (n x hb); the G1 parts along n and v are per-point constants times a
 //   scalar sum (Sb), applied once at the end.
 template <int V, bool DIFF, bool SPEC>
-__device__ __forceinline__ void backward_tile(const Problem& pr, const Tiling& tl, int64_t tile) {
+__device__ __forceinline__ void backward_tile(const Problem& pr, const Tiling& tl, int64_t tile,
+                                              const int32_t* order) {
   extern __shared__ float4 smem[];
   constexpr bool DOUT = (V == kV2 || V == kVis);
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
   const int64_t q0 = tile * 64 + lane;
   const bool valid = q0 < pr.P;
-  const int64_t p = (pr.order != nullptr && valid) ? (int64_t)pr.order[q0] : q0;
+  const int64_t p = (order != nullptr && valid) ? (int64_t)order[q0] : q0;
 
   PointState<V, SPEC> st;
   load_point<V, SPEC>(pr, p, valid, st);
@@ -662,14 +665,14 @@ __device__ __forceinline__ void backward_tile(const Problem& pr, const Tiling& t
 
 template <int V, bool DIFF, bool SPEC>
 __global__ __launch_bounds__(1024) void shade_forward_kernel(Problem pr, Tiling tl) {
-  forward_tile<V, DIFF, SPEC>(pr, tl, blockIdx.x);
+  forward_tile<V, DIFF, SPEC>(pr, tl, blockIdx.x, pr.order);
 }
 
 // Three waves per SIMD (<= 168 VGPRs): the measured best for this kernel.
 template <int V, bool DIFF, bool SPEC>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void shade_backward_kernel(
     Problem pr, Tiling tl) {
-  backward_tile<V, DIFF, SPEC>(pr, tl, blockIdx.x);
+  backward_tile<V, DIFF, SPEC>(pr, tl, blockIdx.x, pr.order);
 }
 
 // ---- batched launches: persistent workgroups pull tiles of many problems ------
@@ -709,7 +712,7 @@ __global__ __launch_bounds__(512) void batch_forward_kernel(const BatchHeader* _
     if (g >= total) break;
     const int s = __builtin_amdgcn_readfirstlane(
         find_set(n, g, [&](int i) { return sets[i].tile0; }));
-    forward_tile<V, DIFF, SPEC>(sets[s].pr, sets[s].tl_fwd, g - sets[s].tile0);
+    forward_tile<V, DIFF, SPEC>(sets[s].pr, sets[s].tl_fwd, g - sets[s].tile0, sets[s].order_fwd);
   }
 }
 
@@ -726,7 +729,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(3))) void b
     if (g >= total) break;
     const int s = __builtin_amdgcn_readfirstlane(
         find_set(n, g, [&](int i) { return sets[i].tile0; }));
-    backward_tile<V, DIFF, SPEC>(sets[s].pr, sets[s].tl_bwd, g - sets[s].tile0);
+    backward_tile<V, DIFF, SPEC>(sets[s].pr, sets[s].tl_bwd, g - sets[s].tile0, sets[s].order_bwd);
   }
 }
 

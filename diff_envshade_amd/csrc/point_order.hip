@@ -92,10 +92,13 @@ __device__ __forceinline__ uint64_t lanemask_lt(int lane) {
 }
 
 // 1. keys, in-block ranks, per-block key counts
-__global__ __launch_bounds__(kThreads) void order_rank_kernel(const float* __restrict__ normal,
-                                                              int64_t P, uint32_t* __restrict__ keyrank,
-                                                              uint32_t* __restrict__ count, int nblk,
-                                                              int nblk_pad) {
+__device__ __forceinline__ void rank_block(const SortJob& job, int blk) {
+  const float* __restrict__ normal = job.normal;
+  const int64_t P = job.P;
+  uint32_t* __restrict__ keyrank = job.keyrank;
+  uint32_t* __restrict__ count = job.count;
+  const int nblk = job.nblk;
+  const int nblk_pad = job.nblk_pad;
   // wcount[k][w] = (round + 1) << 12 | n: first the number n of key-k points of
   // wave w in that round, then (after the column scan) the number of key-k
   // points of this block that precede wave w's.  Key-major, so that thread k
@@ -107,7 +110,6 @@ __global__ __launch_bounds__(kThreads) void order_rank_kernel(const float* __res
   uint16_t* const col = wcount + tid * kWaves;  // 32 bytes, 16-byte aligned
   const int lane = tid & 63;
   const int wv = tid >> 6;
-  const int blk = blockIdx.x;
   // all four rounds' normals are requested before any is used: one memory
   // latency per block instead of one per round
   float nrm[kRounds][3];
@@ -197,11 +199,13 @@ __device__ __forceinline__ uint32_t wave_scan(uint32_t x, int lane) {
 }
 
 // 2. per key: exclusive scan of count[key][0..nblk) in place, total[key]
-__global__ __launch_bounds__(256) void order_scan_kernel(uint32_t* __restrict__ count,
-                                                         uint32_t* __restrict__ total, int nblk,
-                                                         int nblk_pad) {
+__device__ __forceinline__ void scan_group(const SortJob& job, int group) {
+  uint32_t* __restrict__ count = job.count;
+  uint32_t* __restrict__ total = job.total;
+  const int nblk = job.nblk;
+  const int nblk_pad = job.nblk_pad;
   const int lane = threadIdx.x & 63;
-  const int key = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int key = group * 4 + (threadIdx.x >> 6);
   if (key >= kKeys) return;
   uint32_t* row = count + (int64_t)key * nblk_pad;
   uint32_t carry = 0u;
@@ -218,16 +222,19 @@ __global__ __launch_bounds__(256) void order_scan_kernel(uint32_t* __restrict__ 
 // 3. key bases, then scatter.  fused: count holds the raw per-block counts
 // (rows padded with zeros to nblk_pad, a multiple of 4); otherwise the
 // exclusive prefixes of step 2 and total[] the key totals.
-__global__ __launch_bounds__(kThreads) void order_scatter_kernel(
-    const uint32_t* __restrict__ keyrank, const uint32_t* __restrict__ count,
-    const uint32_t* __restrict__ total, int64_t P, int nblk_pad, int fused,
-    int32_t* __restrict__ order) {
+__device__ __forceinline__ void scatter_block(const SortJob& job, int blk) {
+  const uint32_t* __restrict__ keyrank = job.keyrank;
+  const uint32_t* __restrict__ count = job.count;
+  const uint32_t* __restrict__ total = job.total;
+  const int64_t P = job.P;
+  const int nblk_pad = job.nblk_pad;
+  const int fused = job.fused;
+  int32_t* __restrict__ order = job.order;
   __shared__ uint32_t off[kKeys];
   __shared__ uint32_t wsum[kWaves];
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wv = tid >> 6;
-  const int blk = blockIdx.x;
   uint32_t kr[kRounds];
 #pragma unroll
   for (int r = 0; r < kRounds; ++r) {
@@ -263,6 +270,60 @@ __global__ __launch_bounds__(kThreads) void order_scatter_kernel(
   }
 }
 
+// One point set: the job by value, one workgroup per block (per 4 keys for
+// the scan).
+__global__ __launch_bounds__(kThreads) void order_rank_kernel(SortJob job) {
+  rank_block(job, blockIdx.x);
+}
+__global__ __launch_bounds__(256) void order_scan_kernel(SortJob job) { scan_group(job, blockIdx.x); }
+__global__ __launch_bounds__(kThreads) void order_scatter_kernel(SortJob job) {
+  scatter_block(job, blockIdx.x);
+}
+
+// Many point sets in one launch (a batch): work item b belongs to the last
+// job whose first item (blk0, or scan0 for the scan) is at most b.  The job
+// table and the item totals live in device memory, so a fixed grid of
+// persistent workgroups serves any batch (a captured graph replays for the
+// next batch prepared into the same workspace).
+template <bool SCAN>
+__device__ __forceinline__ int find_job(const SortJob* jobs, int njobs, int b) {
+  int lo = 0, hi = njobs - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    const int first = SCAN ? jobs[mid].scan0 : jobs[mid].blk0;
+    if (first <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+__global__ __launch_bounds__(kThreads) void order_rank_jobs_kernel(const SortJobs* const* tabp) {
+  const SortJobs* tab = *tabp;
+  const int njobs = tab->njobs, nitems = tab->nblocks;
+  for (int b = blockIdx.x; b < nitems; b += gridDim.x) {
+    const SortJob job = tab->jobs[find_job<false>(tab->jobs, njobs, b)];
+    __syncthreads();  // the previous item's LDS is free
+    rank_block(job, b - job.blk0);
+  }
+}
+__global__ __launch_bounds__(256) void order_scan_jobs_kernel(const SortJobs* const* tabp) {
+  const SortJobs* tab = *tabp;
+  const int njobs = tab->njobs, nitems = tab->nscan;
+  for (int b = blockIdx.x; b < nitems; b += gridDim.x) {
+    const SortJob job = tab->jobs[find_job<true>(tab->jobs, njobs, b)];
+    scan_group(job, b - job.scan0);
+  }
+}
+__global__ __launch_bounds__(kThreads) void order_scatter_jobs_kernel(const SortJobs* const* tabp) {
+  const SortJobs* tab = *tabp;
+  const int njobs = tab->njobs, nitems = tab->nblocks;
+  for (int b = blockIdx.x; b < nitems; b += gridDim.x) {
+    const SortJob job = tab->jobs[find_job<false>(tab->jobs, njobs, b)];
+    __syncthreads();
+    scatter_block(job, b - job.blk0);
+  }
+}
+
 }  // namespace
 
 OrderLayout order_layout(int64_t P) {
@@ -279,29 +340,49 @@ OrderLayout order_layout(int64_t P) {
   return L;
 }
 
-hipError_t launch_point_order(const float* normal, int64_t P, void* ws, hipStream_t stream) {
-  if (P <= 0) return hipSuccess;
+SortJob sort_job(const float* normal, int64_t P, void* ws) {
+  SortJob job{};
+  if (P <= 0) return job;
   const OrderLayout L = order_layout(P);
   char* base = static_cast<char*>(ws);
-  int32_t* order = reinterpret_cast<int32_t*>(base + L.order_off);
-  uint32_t* keyrank = reinterpret_cast<uint32_t*>(base + L.keyrank_off);
-  uint32_t* count = reinterpret_cast<uint32_t*>(base + L.count_off);
-  uint32_t* total = reinterpret_cast<uint32_t*>(base + L.total_off);
-  const int nblk = (int)((P + kOrderBlock - 1) / kOrderBlock);
-  const int nblk_pad = (nblk + 3) & ~3;
-  const bool fused = nblk <= kFusedBlocks;
-  hipLaunchKernelGGL(order_rank_kernel, dim3(nblk), dim3(kThreads), 0, stream, normal, P, keyrank,
-                     count, nblk, nblk_pad);
+  job.normal = normal;
+  job.P = P;
+  job.order = reinterpret_cast<int32_t*>(base + L.order_off);
+  job.keyrank = reinterpret_cast<uint32_t*>(base + L.keyrank_off);
+  job.count = reinterpret_cast<uint32_t*>(base + L.count_off);
+  job.total = reinterpret_cast<uint32_t*>(base + L.total_off);
+  job.nblk = (int)((P + kOrderBlock - 1) / kOrderBlock);
+  job.nblk_pad = (job.nblk + 3) & ~3;
+  job.fused = job.nblk <= kFusedBlocks ? 1 : 0;
+  return job;
+}
+
+int sort_scan_groups(const SortJob& job) { return job.fused ? 0 : (kKeys + 3) / 4; }
+
+hipError_t launch_point_order(const float* normal, int64_t P, void* ws, hipStream_t stream) {
+  if (P <= 0) return hipSuccess;
+  const SortJob job = sort_job(normal, P, ws);
+  hipLaunchKernelGGL(order_rank_kernel, dim3(job.nblk), dim3(kThreads), 0, stream, job);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  if (!fused) {
-    hipLaunchKernelGGL(order_scan_kernel, dim3((kKeys + 3) / 4), dim3(256), 0, stream, count,
-                       total, nblk, nblk_pad);
+  if (!job.fused) {
+    hipLaunchKernelGGL(order_scan_kernel, dim3(sort_scan_groups(job)), dim3(256), 0, stream, job);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
-  hipLaunchKernelGGL(order_scatter_kernel, dim3(nblk), dim3(kThreads), 0, stream, keyrank, count,
-                     total, P, nblk_pad, fused ? 1 : 0, order);
+  hipLaunchKernelGGL(order_scatter_kernel, dim3(job.nblk), dim3(kThreads), 0, stream, job);
+  return hipGetLastError();
+}
+
+hipError_t launch_point_order_jobs(const SortJobs* const* tabp, int max_groups,
+                                   hipStream_t stream) {
+  hipLaunchKernelGGL(order_rank_jobs_kernel, dim3(max_groups), dim3(kThreads), 0, stream, tabp);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(order_scan_jobs_kernel, dim3(4 * max_groups), dim3(256), 0, stream, tabp);
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(order_scatter_jobs_kernel, dim3(max_groups), dim3(kThreads), 0, stream, tabp);
   return hipGetLastError();
 }
 

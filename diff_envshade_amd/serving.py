@@ -49,24 +49,38 @@ class Request:
 
 
 REDUCE_ROWS = 16  # partial rows per stage-1 group of the env-gradient reduce
+ORDER_MIN_PAIRS = 1 << 24  # sets of this many point-texel pairs are sorted (backward policy)
+ORDER_BLOCK = 1024  # points per block of the counting sort
+ORDER_KEYS = 1024  # keys (octahedral cells) of the counting sort
 
 
 def _align256(x: int) -> int:
     return (x + 255) // 256 * 256
 
 
+def _order_bytes(points: int) -> int:
+    """The point-order workspace of a sorted set (order, key/rank words, per-key
+    block counts, key totals; the library's order_layout)."""
+    nblk_pad = (-(-points // ORDER_BLOCK) + 3) & ~3
+    return 2 * _align256(points * 4) + _align256(ORDER_KEYS * nblk_pad * 4) + \
+        _align256(ORDER_KEYS * 4)
+
+
 def request_bytes(points: int, texels: int, backward: bool, variant: str = "vis") -> int:
-    """Device bytes a request holds while in flight: its outputs, plus the
-    backward gradients and env-map-gradient workspace."""
+    """Device bytes a request holds while in flight: its outputs, its share of
+    the batch workspace (env-map-gradient partials, and the point order of a set
+    large enough to be sorted), plus the backward gradients."""
     out = points * 3 * 4 * (2 if variant in ("v2", "vis") else 1)
-    if not backward:
-        return out
     tiles = (points + 63) // 64
-    grads = points * (3 + 3 + 3 + 3 + 1) * 4 + 3 * texels * 4
     # per-tile fp32 partials, then fp64 sums of REDUCE_ROWS tiles each, both
-    # 256-byte aligned (the library's des_backward_workspace_size)
+    # 256-byte aligned (the library's batch workspace holds them for every set)
     ws = _align256(tiles * 3 * texels * 4) + \
         _align256(((tiles + REDUCE_ROWS - 1) // REDUCE_ROWS) * 3 * texels * 8)
+    if points * texels >= ORDER_MIN_PAIRS:
+        ws += _order_bytes(points)
+    if not backward:
+        return out + ws
+    grads = points * (3 + 3 + 3 + 3 + 1) * 4 + 3 * texels * 4
     return out + grads + ws
 
 

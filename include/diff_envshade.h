@@ -213,10 +213,16 @@ DES_API int des_set_tiling_override(int32_t ntc_forward, int32_t ntc_backward);
  *   2. des_batch_prepare  -> validates the sets, writes the descriptor table
  *                            into `staging` and enqueues its upload into the
  *                            workspace on `stream` (keep `staging` alive, and
- *                            preferably pinned, until the copy has run)
+ *                            preferably pinned, until the copy has run), then
+ *                            the point order of every set large enough to be
+ *                            sorted (the single-set policy, des_set_point_order)
  *   3. des_batch_forward / des_batch_backward -> persistent launches that read
  *      every size from the workspace; they can be captured once in a HIP graph
  *      and replayed for any later batch prepared into the same workspace.
+ * A set's forward and backward shade in the order computed at prepare time;
+ * des_batch_sort recomputes it from the normals' current values (any order
+ * gives correct results, a stale one only skips fewer texels), and is
+ * capturable like the launches.
  */
 typedef struct des_set {
   int64_t P;
@@ -273,6 +279,11 @@ DES_API int des_batch_forward(int32_t variant, int enable_diffuse, int enable_sp
 /* Does nothing on the device when the batch was prepared without with_backward. */
 DES_API int des_batch_backward(int32_t variant, int enable_diffuse, int enable_specular,
                                int32_t ntc_backward, void* workspace, void* stream);
+
+/* Recomputes the point order of the prepared batch's sorted sets (three
+ * persistent launches that read the job table from the workspace; a batch with
+ * no sorted set costs three near-empty launches). */
+DES_API int des_batch_sort(void* workspace, void* stream);
 
 /* ---- many env maps lighting one point set, on MFMA (no counterpart in the reference) ----
  * shading[p, e, c] and diffuse_shading[p, e, c] (P x E x 3) for E env maps

@@ -70,6 +70,34 @@ def test_serving_charges_the_batch_workspace(P, H, W):
     grads = P * 13 * 4 + 3 * H * W * 4
     outs = P * 3 * 4 * 2
     assert request_bytes(P, H * W, True) - grads - outs == ws.value - st.value
+    assert request_bytes(P, H * W, False) - outs == ws.value - st.value
+
+
+def test_batch_workspace_holds_the_order_of_large_sets():
+    # a set of >= 2^24 point-texel pairs gets a point-order workspace (sorted at
+    # prepare time); smaller sets do not
+    from diff_envshade_amd.batch import DesSet, _bind
+    L = _bind(_lib.load())
+
+    def plan(*specs):
+        arr = (DesSet * len(specs))()
+        for a, (P, H, W) in zip(arr, specs):
+            a.P, a.H, a.W = P, H, W
+        ws, st = ctypes.c_size_t(0), ctypes.c_size_t(0)
+        assert L.des_batch_plan(4, arr, len(specs), ctypes.byref(ws), ctypes.byref(st)) == 0
+        return ws.value, st.value
+
+    small, big = (20000, 16, 32), (70000, 16, 32)
+    ws_s, st_s = plan(small)
+    ws_b, st_b = plan(big)
+    ws_sb, st_sb = plan(small, big)
+    try:
+        _lib.set_point_order(-1)
+        ws_b0, st_b0 = plan(big)
+    finally:
+        _lib.set_point_order(0)
+    assert ws_b0 - st_b0 < ws_b - st_b  # the order region
+    assert ws_sb - st_sb == (ws_s - st_s) + (ws_b - st_b)
 
 
 def test_forward_workspace_follows_point_order_policy():

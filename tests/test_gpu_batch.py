@@ -73,7 +73,20 @@ def test_batch_results_independent_of_batch_mates():
             assert np.array_equal(x, y)
 
 
-def test_graph_capture_and_replay_new_batches():
+@pytest.mark.parametrize("order", [0, 1], ids=["order-auto", "sorted"])
+def test_graph_capture_and_replay_new_batches(order):
+    # the graph holds sort + forward + backward; every size is read from the
+    # workspace, so it replays for new batches (sorted ones too, order=1)
+    from diff_envshade_amd import _lib
+    from diff_envshade_amd.batch import ShadeBatch
+    _lib.set_point_order(order)
+    try:
+        _replay_new_batches()
+    finally:
+        _lib.set_point_order(0)
+
+
+def _replay_new_batches():
     from diff_envshade_amd.batch import ShadeBatch
     nps, devs = _sets([(4000, 32, 64), (3000, 16, 32), (100, 16, 32)], seed=40)
     b = ShadeBatch("vis")
@@ -157,3 +170,52 @@ def test_batch_other_variants(variant):
                      "grad_roughness", "grad_envmap")
         bwd = [o[k].cpu().numpy() for k in names]
         compare(variant, (fwd, bwd), oracle_run(variant, d))
+
+
+@pytest.mark.parametrize("ntc_f,ntc_b", [(4, 4), (8, 2)])
+def test_sorted_batch_matches_oracle_and_single_set(ntc_f, ntc_b):
+    # with the point order forced on, prepare sorts every set once and both
+    # directions shade in that order, as the single-set entry points do
+    from diff_envshade_amd import _lib
+    from diff_envshade_amd.batch import ShadeBatch
+    nps, devs = _sets(SPECS, seed=40)
+    try:
+        _lib.set_point_order(1)
+        b = ShadeBatch("vis", enable_specular=True, ntc_forward=ntc_f, ntc_backward=ntc_b)
+        outs = b.prepare(devs, with_backward=True)
+        b.forward()
+        b.backward()
+        torch.cuda.synchronize()
+        _lib.set_tiling_override(ntc_f, ntc_b)
+        for d, o in zip(nps, outs):
+            got = _np(o)
+            if d["normal"].shape[0] > 0:
+                compare("vis", got, oracle_run("vis", d))
+            single = gpu_run("vis", d)
+            for a, s in zip(got[0] + got[1], single[0] + single[1]):
+                assert np.array_equal(a.reshape(s.shape), s)
+    finally:
+        _lib.set_tiling_override(0, 0)
+        _lib.set_point_order(0)
+
+
+def test_automatic_order_policy_in_a_batch():
+    # one set above the backward's sort threshold (70k x 512 pairs) beside small
+    # ones: the batch sorts it for the backward only, like the single-set calls
+    from diff_envshade_amd.batch import ShadeBatch
+    nps, devs = _sets([(70000, 16, 32), (3000, 16, 32)], seed=50)
+    b = ShadeBatch("vis", enable_specular=True, ntc_forward=4, ntc_backward=4)
+    outs = b.prepare(devs, with_backward=True)
+    b.forward()
+    b.backward()
+    torch.cuda.synchronize()
+    from diff_envshade_amd import _lib
+    try:
+        _lib.set_tiling_override(4, 4)
+        for d, o in zip(nps, outs):
+            got = _np(o)
+            single = gpu_run("vis", d)
+            for a, s in zip(got[0] + got[1], single[0] + single[1]):
+                assert np.array_equal(a.reshape(s.shape), s)
+    finally:
+        _lib.set_tiling_override(0, 0)
