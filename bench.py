@@ -77,12 +77,15 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--workload", default="dropin",
-                    choices=["dropin", "batched", "relight", "serving", "copy"],
+                    choices=["dropin", "batched", "relight", "serving", "copy", "nearest_mesh",
+                             "bitfield"],
                     help="dropin (default, the headline); batched: 48 uneven point sets in one "
                          "persistent launch; relight: 64 env maps x 262,144 points on MFMA; "
                          "serving: Poisson arrivals through the continuous-batching server; "
                          "copy: a plain 256 MiB device copy, the calibration kernel for the "
-                         "HBM byte counters")
+                         "HBM byte counters; nearest_mesh: nearest mesh points forward+backward "
+                         "for 70,000 queries; bitfield: decoding the headline's 512-bit "
+                         "visibility words to floats")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the shader has no "
@@ -90,7 +93,8 @@ def main() -> int:
         return 2
     if args.workload != "dropin":
         line = {"batched": bench_batched, "relight": bench_relight,
-                "serving": bench_serving, "copy": bench_copy}[args.workload](args)
+                "serving": bench_serving, "copy": bench_copy,
+                "nearest_mesh": bench_nearest_mesh, "bitfield": bench_bitfield}[args.workload](args)
         print(json.dumps(line), flush=True)
         return 0
 
@@ -288,6 +292,205 @@ def _time_ms(fn, reps=10, warmup=2):
     torch.cuda.synchronize()
     ts = sorted(a.elapsed_time(b) for a, b in evs)
     return ts[len(ts) // 2], ts
+
+
+def _queued_avg_ms(fn, n):
+    """Average GPU time per call of fn() over n calls queued behind a spin
+    kernel, so that the GPU runs them back to back with no launch gaps (HIP
+    events on the current stream, the one the calls launch on)."""
+    import torch
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    fn()
+    torch.cuda.synchronize()
+    torch.cuda._sleep(50_000_000)  # longer than the host needs to enqueue the n calls
+    a.record()
+    for _ in range(n):
+        fn()
+    b.record()
+    torch.cuda.synchronize()
+    return a.elapsed_time(b) / n
+
+
+# FP64 operations per candidate face of the nearest-mesh-point search, counted
+# on the closest-point formulas (etc.cu:302-393) with add/sub/mul/div = 1 and
+# the region compares excluded: edges and query offset 9, the five dot
+# products 25, det, s and t 9, the interior (s, t) 3, the closest point 12,
+# its offset and squared length 8.
+FLOP_NEAREST_PER_CANDIDATE = 66
+PEAK_FP64_FLOPS = 78.6e12  # MI355X FP64 vector peak
+
+
+def bench_nearest_mesh(args):
+    """Nearest mesh points forward + backward (get_nearest_mesh_points_*):
+    70,000 queries at radius 1 + N(0, 0.05) around a closed 20,480-face surface
+    (icosphere of subdivision 5, vertices jittered radially by 2%), the faces
+    around each query's 8 nearest vertices (k-d tree) as candidates."""
+    import numpy as np
+    import torch
+
+    from diff_envshade_amd import _diff_envshade as C
+    from diff_envshade_amd import workloads as WL
+    n, K = args.points, 8
+    d = WL.mesh_queries(seed=0, subdiv=5, n=n, K=K)
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    t = {k: torch.from_numpy(v).to(dev) for k, v in d.items()}
+    rng = np.random.default_rng(1)
+    g_np = rng.normal(size=(3, n)).astype(np.float32)
+    g = torch.from_numpy(g_np).to(dev)
+    mesh = [t["adjacency_head"], t["adjacency_list"], t["face_vertex_list"], t["vertex_pos"],
+            t["query_pos"], t["idxs"]]
+
+    def fwd():
+        return C.get_nearest_mesh_points_forward(*mesh)
+
+    tri0, _ = fwd()
+
+    def bwd():
+        return C.get_nearest_mesh_points_backward(t["face_vertex_list"], t["vertex_pos"],
+                                                  t["query_pos"], tri0, g)
+
+    def step():
+        tri, bary = fwd()
+        return tri, bary, C.get_nearest_mesh_points_backward(
+            t["face_vertex_list"], t["vertex_pos"], t["query_pos"], tri, g)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    steps = args.steps
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        out = step()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    reps = min(steps, 200)
+    fwd_ms = _queued_avg_ms(fwd, reps)
+    bwd_ms = _queued_avg_ms(bwd, reps)
+    print(f"[bench] nearest mesh: {ms:.4f} ms/step, forward kernel {fwd_ms * 1e3:.2f} us, "
+          f"backward {bwd_ms * 1e3:.2f} us", file=sys.stderr, flush=True)
+
+    head, idxs = d["adjacency_head"], d["idxs"]
+    valid = (idxs >= 0) & (idxs < head.shape[1])
+    candidates = int(head[0][idxs[valid]].sum())
+    flops = candidates * FLOP_NEAREST_PER_CANDIDATE
+    achieved = flops / (fwd_ms * 1e-3)
+    # compulsory HBM bytes of one forward launch: queries, idxs and outputs once,
+    # the mesh arrays once
+    mesh_bytes = sum(d[k].nbytes for k in ("adjacency_head", "adjacency_list", "face_vertex_list",
+                                           "vertex_pos"))
+    fwd_bytes = d["query_pos"].nbytes + idxs.nbytes + n * (8 + 12) + mesh_bytes
+    line = {
+        "metric": "nearest-mesh-point queries/s, forward+backward",
+        "value": n / (ms * 1e-3), "unit": "queries/s", "n_gpus": 1, "steps": steps,
+        "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"{n} queries around a 20,480-face closed surface (10,242 "
+                               f"vertices), faces around the {K} nearest vertices, "
+                               "get_nearest_mesh_points_forward + _backward",
+                   "queries": n, "K": K, "faces": int(d["face_vertex_list"].shape[1]),
+                   "candidates_per_launch": candidates},
+        "kernels_us": {"forward": fwd_ms * 1e3, "backward": bwd_ms * 1e3,
+                       "note": "average over back-to-back calls queued behind a spin kernel"},
+        "roofline": {"bound": "fp64", "kernel": "nearest_forward_kernel",
+                     "achieved": achieved / 1e12, "peak": PEAK_FP64_FLOPS / 1e12,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_FLOPS,
+                     "flop_per_candidate": FLOP_NEAREST_PER_CANDIDATE,
+                     "traffic": None,
+                     "hbm_compulsory_bytes": fwd_bytes,
+                     "hbm_frac": fwd_bytes / (fwd_ms * 1e-3) / PEAK_HBM_BYTES},
+    }
+    if not (args.no_accuracy and args.no_cpu_baseline):
+        O = _oracle()
+        tri, bary, gq = (x.cpu().numpy() for x in out)
+        runs = []
+        for _ in range(1 if args.no_cpu_baseline else 6):
+            a = time.perf_counter()
+            rtri, rbary = O.nearest_mesh_points_forward(*(d[k] for k in (
+                "adjacency_head", "adjacency_list", "face_vertex_list", "vertex_pos", "query_pos",
+                "idxs")))
+            rg = O.nearest_mesh_points_backward(d["face_vertex_list"], d["vertex_pos"],
+                                                d["query_pos"], rtri, g_np)
+            runs.append(time.perf_counter() - a)
+        line["exact_vs_fp64_oracle"] = {
+            "triangle_ids_differing": int((tri != rtri).sum()),
+            "barycentric_max_abs_diff": float(np.abs(bary - rbary.astype(np.float32)).max()),
+            "grad_query_max_abs_diff": float(np.abs(gq - rg.astype(np.float32)).max()),
+        }
+        if not args.no_cpu_baseline:
+            timed = sorted(runs[1:])
+            med = timed[len(timed) // 2]
+            line["cpu_baseline"] = {"value": n / med, "unit": "queries/s", "cores": O.max_threads(),
+                                    "kind": "port",
+                                    "sample": f"the full workload ({n} queries, forward+backward, "
+                                              f"FP64 oracle, OpenMP), median of {len(timed)} "
+                                              "runs after 1 warm-up",
+                                    "seconds_per_step": med}
+    return line
+
+
+def bench_bitfield(args):
+    """extract_bitfield_forward over the headline's visibility words: 70,000
+    points x 512 bits (16 slots) decoded to 70,000 x 512 floats."""
+    import numpy as np
+    import torch
+
+    from diff_envshade_amd import _diff_envshade as C
+    from diff_envshade_amd import workloads as WL
+    n, N_bit = args.points, 512
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    d = WL.dropin_headline(0, n=n, H=16, W=32)
+    words = torch.from_numpy(np.ascontiguousarray(d["visibility"])).to(dev)
+
+    def call():
+        return C.extract_bitfield_forward(words, N_bit, 0, n)
+
+    for _ in range(args.warmup):
+        call()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        out = call()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    k_ms = _queued_avg_ms(call, min(args.steps, 200))
+    nbytes = words.numel() * 4 + n * N_bit * 4
+    achieved = nbytes / (k_ms * 1e-3)
+    print(f"[bench] bitfield: {ms:.4f} ms/call, kernel {k_ms * 1e3:.2f} us", file=sys.stderr,
+          flush=True)
+    line = {
+        "metric": "visibility bitfield decode, points/s", "value": n / (ms * 1e-3),
+        "unit": "points/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "u32", "data": "synthetic",
+        "config": {"workload": f"{n} points x {N_bit}-bit visibility (16 x {n} words) -> "
+                               f"{n} x {N_bit} floats, extract_bitfield_forward"},
+        "kernels_us": {"bitfield": k_ms * 1e3},
+        "roofline": {"bound": "hbm", "kernel": "bitfield_kernel", "achieved": achieved / 1e9,
+                     "peak": PEAK_HBM_BYTES / 1e9, "unit": "GB/s",
+                     "frac": achieved / PEAK_HBM_BYTES, "traffic": None,
+                     "bytes_per_launch": nbytes},
+    }
+    if not (args.no_accuracy and args.no_cpu_baseline):
+        O = _oracle()
+        got = out.cpu().numpy()
+        runs = []
+        for _ in range(1 if args.no_cpu_baseline else 4):
+            a = time.perf_counter()
+            ref = O.extract_bitfield(d["visibility"], N_bit, 0, n)
+            runs.append(time.perf_counter() - a)
+        line["exact_vs_oracle"] = {"elements_differing": int((got != ref).sum())}
+        if not args.no_cpu_baseline:
+            timed = sorted(runs[1:])
+            med = timed[len(timed) // 2]
+            line["cpu_baseline"] = {"value": n / med, "unit": "points/s", "cores": 1,
+                                    "kind": "port",
+                                    "sample": f"the full workload ({n} points x {N_bit} bits, "
+                                              "scalar C oracle), median of "
+                                              f"{len(timed)} runs after 1 warm-up",
+                                    "seconds_per_step": med}
+    return line
 
 
 def bench_copy(args):

@@ -14,9 +14,11 @@ float32 (the reference admits half/double in CHECK_IS_FLOATING but then reads
 them as float); grad_envmap is summed in a fixed order and is bitwise
 reproducible.
 
-The ten non-shader bindings of the reference (etc.cu: camera rays, env-map
-blur / sampling, nearest mesh points, rotation conversion, bitfield
-extraction) are outside this library and raise NotImplementedError.
+The mesh utilities that produce the shader's mesh inputs (nearest mesh
+points, closest point on a given triangle, visibility bitfield decoding;
+bindings.cpp:27-33, etc.cu) follow the same conventions.  The other etc.cu
+bindings (camera rays, env-map blur / sampling, rotation conversion) are
+outside this library and raise NotImplementedError.
 """
 from __future__ import annotations
 
@@ -35,6 +37,11 @@ __all__ = [
     "bruteforce_specular_shader_backward2",
     "bruteforce_specularvisibility_shader_forward2",
     "bruteforce_specularvisibility_shader_backward2",
+    "get_nearest_mesh_points_forward",
+    "get_nearest_mesh_points_backward",
+    "get_uv_of_triangle_forward",
+    "get_uv_of_triangle_backward",
+    "extract_bitfield_forward",
 ]
 
 
@@ -387,7 +394,135 @@ def bruteforce_specularvisibility_shader_backward2(grad_shading, grad_diffuse_sh
     return outs
 
 
-# ---- reference bindings outside this library (bindings.cpp:21-33, etc.cu) ------------
+# ---- mesh utilities (bindings.cpp:27-33, etc.cu) -------------------------------------
+
+def _mesh_device(query_pos, name="query_pos"):
+    _check_float(query_pos, name)
+    return query_pos.device
+
+
+def get_nearest_mesh_points_forward(adjacency_head, adjacency_list, face_vertex_list, vertex_pos,
+                                    query_pos, idxs):
+    """Nearest point on the faces around each query point's K nearest vertices
+    (etc.cu:1277-1326): returns [nearest_triangle_id (P int64),
+    barycentric_coord (3 x P)].  adjacency_head 2 x N_vertex (face counts,
+    then their inclusive prefix sum), adjacency_list int32, face_vertex_list
+    3 x N_face, vertex_pos 3 x N_vertex, query_pos 3 x P, idxs P x K int64."""
+    dev = _mesh_device(query_pos)
+    for t, n in ((adjacency_head, "adjacency_head"), (adjacency_list, "adjacency_list"),
+                 (face_vertex_list, "face_vertex_list")):
+        _check_int(t, n)
+    _check_float(vertex_pos, "vertex_pos")
+    _check_long(idxs, "idxs")
+    if adjacency_head.dim() != 2 or adjacency_head.shape[0] != 2:
+        raise RuntimeError(f"adjacency_head must have shape (2, N_vertex), got {tuple(adjacency_head.shape)}")
+    NV = int(adjacency_head.shape[1])
+    if face_vertex_list.dim() != 2 or face_vertex_list.shape[0] != 3:
+        raise RuntimeError(f"face_vertex_list must have shape (3, N_face), got {tuple(face_vertex_list.shape)}")
+    NF = int(face_vertex_list.shape[1])
+    _check_shape(vertex_pos, "vertex_pos", (3, NV))
+    if query_pos.dim() != 2 or query_pos.shape[0] != 3:
+        raise RuntimeError(f"query_pos must have shape (3, P), got {tuple(query_pos.shape)}")
+    P = int(query_pos.shape[1])
+    if idxs.dim() != 2 or idxs.shape[0] != P:
+        raise RuntimeError(f"idxs must have shape (P, K), got {tuple(idxs.shape)}")
+    K = int(idxs.shape[1])
+    _same_device(dev, [("adjacency_head", adjacency_head), ("adjacency_list", adjacency_list),
+                       ("face_vertex_list", face_vertex_list), ("vertex_pos", vertex_pos),
+                       ("idxs", idxs)])
+    with torch.cuda.device(dev):
+        tri = torch.empty(P, dtype=torch.int64, device=dev)
+        bary = torch.empty(3, P, dtype=torch.float32, device=dev)
+        _lib.call("des_get_nearest_mesh_points_forward", NV, NF, P, K, _ptr(adjacency_head),
+                  _ptr(adjacency_list), int(adjacency_list.numel()), _ptr(face_vertex_list),
+                  _ptr(vertex_pos), _ptr(query_pos), _ptr(idxs), _ptr(tri), _ptr(bary),
+                  _stream(dev))
+    return [tri, bary]
+
+
+def get_nearest_mesh_points_backward(face_vertex_list, vertex_pos, query_pos, nearest_triangle_id,
+                                     grad_barycentric_coord):
+    """grad_query_pos (3 x P) of get_nearest_mesh_points_forward's barycentric
+    output (etc.cu:1328-1365), from grad_barycentric_coord (3 x P)."""
+    dev = _mesh_device(query_pos)
+    _check_int(face_vertex_list, "face_vertex_list")
+    _check_float(vertex_pos, "vertex_pos")
+    _check_long(nearest_triangle_id, "nearest_triangle_id")
+    _check_float(grad_barycentric_coord, "grad_barycentric_coord")
+    if face_vertex_list.dim() != 2 or face_vertex_list.shape[0] != 3:
+        raise RuntimeError(f"face_vertex_list must have shape (3, N_face), got {tuple(face_vertex_list.shape)}")
+    if vertex_pos.dim() != 2 or vertex_pos.shape[0] != 3:
+        raise RuntimeError(f"vertex_pos must have shape (3, N_vertex), got {tuple(vertex_pos.shape)}")
+    if query_pos.dim() != 2 or query_pos.shape[0] != 3:
+        raise RuntimeError(f"query_pos must have shape (3, P), got {tuple(query_pos.shape)}")
+    NF, NV, P = int(face_vertex_list.shape[1]), int(vertex_pos.shape[1]), int(query_pos.shape[1])
+    _check_shape(nearest_triangle_id, "nearest_triangle_id", (P,))
+    _check_shape(grad_barycentric_coord, "grad_barycentric_coord", (3, P))
+    _same_device(dev, [("face_vertex_list", face_vertex_list), ("vertex_pos", vertex_pos),
+                       ("nearest_triangle_id", nearest_triangle_id),
+                       ("grad_barycentric_coord", grad_barycentric_coord)])
+    with torch.cuda.device(dev):
+        g = torch.empty(3, P, dtype=torch.float32, device=dev)
+        _lib.call("des_get_nearest_mesh_points_backward", NV, NF, P, _ptr(face_vertex_list),
+                  _ptr(vertex_pos), _ptr(query_pos), _ptr(nearest_triangle_id),
+                  _ptr(grad_barycentric_coord), _ptr(g), _stream(dev))
+    return g
+
+
+def _uv_checks(triangles, query_pos):
+    dev = _mesh_device(query_pos)
+    _check_float(triangles, "triangles")
+    if query_pos.dim() != 2 or query_pos.shape[1] != 3:
+        raise RuntimeError(f"query_pos must have shape (P, 3), got {tuple(query_pos.shape)}")
+    P = int(query_pos.shape[0])
+    _check_shape(triangles, "triangles", (P, 3, 3))
+    _same_device(dev, [("triangles", triangles)])
+    return dev, P
+
+
+def get_uv_of_triangle_forward(triangles, query_pos):
+    """Barycentric coordinates (P x 3: 1 - s - t, s, t) of the closest point of
+    triangle p (triangles P x 3 x 3) to query point p (P x 3), etc.cu:1218-1244."""
+    dev, P = _uv_checks(triangles, query_pos)
+    with torch.cuda.device(dev):
+        bary = torch.empty(P, 3, dtype=torch.float32, device=dev)
+        _lib.call("des_get_uv_of_triangle_forward", P, _ptr(triangles), _ptr(query_pos),
+                  _ptr(bary), _stream(dev))
+    return bary
+
+
+def get_uv_of_triangle_backward(triangles, query_pos, grad_barycentric_coord):
+    """grad_query_pos (P x 3) of get_uv_of_triangle_forward, etc.cu:1246-1275."""
+    dev, P = _uv_checks(triangles, query_pos)
+    _check_float(grad_barycentric_coord, "grad_barycentric_coord")
+    _check_shape(grad_barycentric_coord, "grad_barycentric_coord", (P, 3))
+    _same_device(dev, [("grad_barycentric_coord", grad_barycentric_coord)])
+    with torch.cuda.device(dev):
+        g = torch.empty(P, 3, dtype=torch.float32, device=dev)
+        _lib.call("des_get_uv_of_triangle_backward", P, _ptr(triangles), _ptr(query_pos),
+                  _ptr(grad_barycentric_coord), _ptr(g), _stream(dev))
+    return g
+
+
+def extract_bitfield_forward(bitfield, N_bit, i_start, i_end):
+    """(i_end - i_start) x N_bit floats, [b][k] = bit k % 32 of word
+    [k / 32][i_start + b] of bitfield (N_slot x N int32), etc.cu:1044-1071."""
+    _check_int(bitfield, "bitfield")
+    if bitfield.dim() != 2:
+        raise RuntimeError(f"bitfield must have shape (N_slot, N), got {tuple(bitfield.shape)}")
+    dev = bitfield.device
+    N_slot, N = int(bitfield.shape[0]), int(bitfield.shape[1])
+    N_bit, i_start, i_end = int(N_bit), int(i_start), int(i_end)
+    valid = 0 <= i_start <= i_end <= N and 0 <= N_bit <= 32 * N_slot
+    with torch.cuda.device(dev):
+        # on invalid ranges the library reports the error (no output is needed)
+        out = torch.empty(i_end - i_start, N_bit, dtype=torch.float32, device=dev) if valid else None
+        _lib.call("des_extract_bitfield_forward", _ptr(bitfield), N_slot, N, N_bit, i_start, i_end,
+                  _ptr(out), _stream(dev))
+    return out
+
+
+# ---- reference bindings outside this library (bindings.cpp:20-24, 31, etc.cu) ---------
 
 def _not_here(name):
     def fn(*args, **kwargs):
@@ -400,9 +535,4 @@ def _not_here(name):
 blur_envmap_forward = _not_here("blur_envmap_forward")
 sample_envmap_forward = _not_here("sample_envmap_forward")
 generate_camera_ray_forward = _not_here("generate_camera_ray_forward")
-get_nearest_mesh_points_forward = _not_here("get_nearest_mesh_points_forward")
-get_nearest_mesh_points_backward = _not_here("get_nearest_mesh_points_backward")
-get_uv_of_triangle_forward = _not_here("get_uv_of_triangle_forward")
-get_uv_of_triangle_backward = _not_here("get_uv_of_triangle_backward")
 mt_format_conversion_forward = _not_here("mt_format_conversion_forward")
-extract_bitfield_forward = _not_here("extract_bitfield_forward")

@@ -44,6 +44,13 @@ _SIGS = {
     "des_bruteforce_specularvisibility_shader_backward2":
         [_i64, _i32, _i32, _i32, _i32, _i32, _i32] + [_vp] * 13 + [_int] + [_vp] * 6
         + [_vp, _sz, _vp],
+    "des_get_nearest_mesh_points_forward":
+        [_i32, _i64, _i64, _i32, _vp, _vp, _i64] + [_vp] * 6 + [_vp],
+    "des_get_nearest_mesh_points_backward":
+        [_i32, _i64, _i64] + [_vp] * 6 + [_vp],
+    "des_get_uv_of_triangle_forward": [_i64] + [_vp] * 3 + [_vp],
+    "des_get_uv_of_triangle_backward": [_i64] + [_vp] * 4 + [_vp],
+    "des_extract_bitfield_forward": [_vp, _i32, _i64, _i32, _i64, _i64, _vp, _vp],
 }
 
 ERROR_NAMES = {

@@ -218,3 +218,78 @@ def uneven_batch(seed: int = 0, n_sets: int = 48, n_min: int = 2000, n_max: int 
         t["visibility"] = make_visibility_device(t["normal"], H, W, seed=seed * 1000 + i)
         sets.append(t)
     return sets
+
+
+# ---- meshes and nearest-mesh-point queries ---------------------------------------------
+
+def icosphere(subdiv: int) -> tuple[np.ndarray, np.ndarray]:
+    """Unit icosphere: vertices (V, 3) float64, faces (F, 3) int32 with
+    V = 10 * 4^subdiv + 2 and F = 20 * 4^subdiv, counter-clockwise outward."""
+    t = (1.0 + math.sqrt(5.0)) / 2.0
+    v = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t),
+         (0, 1, -t), (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    f = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4),
+         (11, 10, 2), (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8),
+         (3, 8, 9), (4, 9, 5), (2, 4, 11), (6, 2, 10), (8, 6, 7), (9, 8, 1)]
+    verts = [np.array(p, dtype=np.float64) / np.linalg.norm(p) for p in v]
+    faces = f
+    for _ in range(subdiv):
+        mid = {}
+
+        def midpoint(a, b):
+            key = (min(a, b), max(a, b))
+            if key not in mid:
+                m = verts[a] + verts[b]
+                verts.append(m / np.linalg.norm(m))
+                mid[key] = len(verts) - 1
+            return mid[key]
+
+        nf = []
+        for a, b, c in faces:
+            ab, bc, ca = midpoint(a, b), midpoint(b, c), midpoint(c, a)
+            nf += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
+        faces = nf
+    return np.stack(verts), np.asarray(faces, dtype=np.int32)
+
+
+def vertex_adjacency(faces: np.ndarray, n_vertex: int) -> tuple[np.ndarray, np.ndarray]:
+    """Faces around each vertex in the reference's layout (etc.cu:779-806):
+    adjacency_head 2 x N_vertex int32 (row 0 the face count of vertex v, row 1
+    its inclusive prefix sum) and adjacency_list int32, vertex v's faces at
+    [head[1][v] - head[0][v], head[1][v]) in increasing face order."""
+    F = faces.shape[0]
+    vid = faces.reshape(-1).astype(np.int64)
+    fid = np.repeat(np.arange(F, dtype=np.int64), 3)
+    order = np.lexsort((fid, vid))
+    count = np.bincount(vid, minlength=n_vertex).astype(np.int32)
+    head = np.stack([count, np.cumsum(count, dtype=np.int64).astype(np.int32)])
+    return head, fid[order].astype(np.int32)
+
+
+def mesh_queries(seed: int = 0, subdiv: int = 5, n: int = 70000, K: int = 8,
+                 jitter: float = 0.02, spread: float = 0.05) -> dict:
+    """Nearest-mesh-point workload: an icosphere of the given subdivision with
+    its vertices moved radially by U[-jitter, jitter] (a non-convex closed
+    surface), n query points at radius 1 + N(0, spread) around it, and the K
+    nearest vertices of each query (idxs, ascending distance, from a k-d
+    tree).  Arrays in the reference's layouts: face_vertex_list 3 x F,
+    vertex_pos 3 x V, query_pos 3 x n, idxs n x K int64."""
+    from scipy.spatial import cKDTree
+    rng = np.random.default_rng(seed)
+    verts, faces = icosphere(subdiv)
+    verts = verts * (1.0 + rng.uniform(-jitter, jitter, size=(verts.shape[0], 1)))
+    verts = verts.astype(np.float32)
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    q = (d * (1.0 + spread * rng.normal(size=(n, 1)))).astype(np.float32)
+    head, adj = vertex_adjacency(faces, verts.shape[0])
+    _, idxs = cKDTree(verts.astype(np.float64)).query(q.astype(np.float64), k=K)
+    idxs = np.asarray(idxs, dtype=np.int64).reshape(n, K)
+    return {
+        "adjacency_head": head,
+        "adjacency_list": adj,
+        "face_vertex_list": np.ascontiguousarray(faces.T),
+        "vertex_pos": np.ascontiguousarray(verts.T),
+        "query_pos": np.ascontiguousarray(q.T),
+        "idxs": idxs,
+    }

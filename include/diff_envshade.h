@@ -317,6 +317,61 @@ DES_API int des_relight_backward_envmap(int64_t P, int32_t E, int32_t H, int32_t
                                         int32_t precision, float* grad_envmaps, void* workspace,
                                         size_t workspace_bytes, void* stream);
 
+/* ---- mesh utilities: the producers of the shader's mesh inputs ----
+ * Same shapes and meaning as the reference's bindings (bindings.cpp:27-33,
+ * prototypes gsface_shader.h:122-172); geometry in FP64 with the reference's
+ * closest-point region rule (etc.cu:205-393), each operation correctly
+ * rounded, so results are reproducible and exactly checkable.  Deliberate
+ * differences: indices out of range are skipped instead of read, a point
+ * with no valid candidate face gets triangle id -1 and zero barycentrics (the
+ * reference traps), and the candidate set is not capped at 128 faces. */
+
+/* get_nearest_mesh_points_forward (etc.cu:1277-1326, kernel etc.cu:779-862):
+ * for each query point (query_pos 3 x P), the nearest point on the faces
+ * around its K nearest vertices (idxs P x K int64, negative = none).
+ * adjacency_head 2 x N_vertex int32: [0][v] = number of faces around v,
+ * [1][v] = inclusive prefix sum of [0], so v's faces are adjacency_list
+ * [head[1][v] - head[0][v], head[1][v]) (adjacency_list has N_adjacency
+ * int32 entries).  Outputs nearest_triangle_id (P int64) and
+ * barycentric_coord (3 x P: 1 - s - t, s, t).  Among faces at equal distance
+ * the first in candidate order (vertex by vertex in idxs order, each vertex's
+ * faces in list order) wins, as in the reference. */
+DES_API int des_get_nearest_mesh_points_forward(
+    int32_t N_vertex, int64_t N_face, int64_t P, int32_t K, const int32_t* adjacency_head,
+    const int32_t* adjacency_list, int64_t N_adjacency, const int32_t* face_vertex_list,
+    const float* vertex_pos, const float* query_pos, const int64_t* idxs,
+    int64_t* nearest_triangle_id, float* barycentric_coord, void* stream);
+
+/* get_nearest_mesh_points_backward (etc.cu:1328-1365, kernel etc.cu:864-920):
+ * grad_query_pos (3 x P) from grad_barycentric_coord (3 x P).  The reference
+ * kernel reads the upstream gradient from its own output buffer
+ * (etc.cu:884-888); this one reads grad_barycentric_coord. */
+DES_API int des_get_nearest_mesh_points_backward(
+    int32_t N_vertex, int64_t N_face, int64_t P, const int32_t* face_vertex_list,
+    const float* vertex_pos, const float* query_pos, const int64_t* nearest_triangle_id,
+    const float* grad_barycentric_coord, float* grad_query_pos, void* stream);
+
+/* get_uv_of_triangle_forward / _backward (etc.cu:1218-1275, kernels
+ * etc.cu:707-777): triangles P x 3 x 3 (vertex, xyz), query_pos P x 3,
+ * barycentric_coord and gradients P x 3. */
+DES_API int des_get_uv_of_triangle_forward(int64_t P, const float* triangles,
+                                           const float* query_pos, float* barycentric_coord,
+                                           void* stream);
+DES_API int des_get_uv_of_triangle_backward(int64_t P, const float* triangles,
+                                            const float* query_pos,
+                                            const float* grad_barycentric_coord,
+                                            float* grad_query_pos, void* stream);
+
+/* extract_bitfield_forward (etc.cu:1044-1071, kernel etc.cu:1021-1040):
+ * output[b][k] = bit k % 32 of word [k / 32][i_start + b] of bitfield
+ * (N_slot x N 32-bit words), as 0.0f / 1.0f, for b < i_end - i_start and
+ * k < N_bit; output is (i_end - i_start) x N_bit.  Needs
+ * 0 <= i_start <= i_end <= N and N_bit <= 32 * N_slot (unchecked in the
+ * reference). */
+DES_API int des_extract_bitfield_forward(const int32_t* bitfield, int32_t N_slot, int64_t N,
+                                         int32_t N_bit, int64_t i_start, int64_t i_end,
+                                         float* output, void* stream);
+
 /* Timing hook for benchmarks (no counterpart in the reference): when set,
  * the calling thread's next *_backward* call records hipEvent_t `start` right
  * before and `stop` right after its main shading kernel, on the call's stream,

@@ -26,8 +26,9 @@ _lib = None
 
 def build(force: bool = False) -> str:
     """Compile the oracle with its Makefile (gcc + OpenMP)."""
-    if force or not os.path.exists(_LIB_PATH) or (
-        os.path.getmtime(_LIB_PATH) < os.path.getmtime(os.path.join(_HERE, "des_oracle.c"))
+    if force or not os.path.exists(_LIB_PATH) or any(
+        os.path.getmtime(_LIB_PATH) < os.path.getmtime(os.path.join(_HERE, src))
+        for src in ("des_oracle.c", "mesh_oracle.c")
     ):
         subprocess.run(["make", "-B" if force else "-s", "-C", _HERE, "liboracle_des.so"],
                        check=True, stdout=subprocess.DEVNULL)
@@ -59,6 +60,20 @@ def lib():
         L.des_oracle_delta_omega.argtypes = [c_int, c_int, c_int]
         L.des_oracle_max_threads.restype = c_int
         L.des_oracle_max_threads.argtypes = []
+        L.des_oracle_closest_point.restype = None
+        L.des_oracle_closest_point.argtypes = [dp, dp, dp, dp, dp, dp]
+        L.des_oracle_uv_of_triangle_forward.restype = c_int
+        L.des_oracle_uv_of_triangle_forward.argtypes = [c_i64, dp, dp, dp]
+        L.des_oracle_uv_of_triangle_backward.restype = c_int
+        L.des_oracle_uv_of_triangle_backward.argtypes = [c_i64, dp, dp, dp, dp]
+        L.des_oracle_nearest_forward.restype = c_int
+        L.des_oracle_nearest_forward.argtypes = [c_int, c_i64, c_i64, c_int, i32p, i32p, c_i64,
+                                                 i32p, dp, dp, i64p, i64p, dp]
+        L.des_oracle_nearest_backward.restype = c_int
+        L.des_oracle_nearest_backward.argtypes = [c_int, c_i64, c_i64, i32p, dp, dp, i64p, dp,
+                                                  dp]
+        L.des_oracle_extract_bitfield.restype = None
+        L.des_oracle_extract_bitfield.argtypes = [u32p, c_i64, c_int, c_i64, c_i64, dp]
         _lib = L
     return _lib
 
@@ -157,4 +172,86 @@ def backward(variant: str, inp: dict, grad_shading, grad_diffuse_shading=None,
                                               "roughness", "envmap")), nthreads)
     if rc != 0:
         raise RuntimeError(f"oracle backward failed ({rc})")
+    return out
+
+
+# ---- mesh utilities (mesh_oracle.c; etc.cu of the reference) ----------------------
+
+def closest_point(x0, x1, x2, q):
+    """(s, t) of the closest point x0 + s (x1 - x0) + t (x2 - x0) to q."""
+    L = lib()
+    a = [_d(v) for v in (x0, x1, x2, q)]
+    s, t = ctypes.c_double(), ctypes.c_double()
+    L.des_oracle_closest_point(*(p for _, p in a), ctypes.byref(s), ctypes.byref(t))
+    return s.value, t.value
+
+
+def uv_of_triangle_forward(triangles, query_pos):
+    """triangles P x 3 x 3, query_pos P x 3 -> barycentric P x 3 (FP64)."""
+    tri, tp = _d(triangles)
+    q, qp = _d(query_pos)
+    P = q.shape[0]
+    out = np.zeros((P, 3))
+    rc = lib().des_oracle_uv_of_triangle_forward(
+        P, tp, qp, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    assert rc == 0
+    return out
+
+
+def uv_of_triangle_backward(triangles, query_pos, grad_barycentric):
+    tri, tp = _d(triangles)
+    q, qp = _d(query_pos)
+    g, gp = _d(grad_barycentric)
+    P = q.shape[0]
+    out = np.zeros((P, 3))
+    rc = lib().des_oracle_uv_of_triangle_backward(
+        P, tp, qp, gp, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    assert rc == 0
+    return out
+
+
+def nearest_mesh_points_forward(adjacency_head, adjacency_list, face_vertex_list, vertex_pos,
+                                query_pos, idxs):
+    """-> (nearest_triangle_id P int64, barycentric_coord 3 x P FP64)."""
+    ah, ahp = _typed(adjacency_head, np.int32, ctypes.c_int32)
+    al, alp = _typed(adjacency_list, np.int32, ctypes.c_int32)
+    fv, fvp = _typed(face_vertex_list, np.int32, ctypes.c_int32)
+    vp, vpp = _d(vertex_pos)
+    q, qp = _d(query_pos)
+    ix, ixp = _typed(idxs, np.int64, ctypes.c_int64)
+    N_vertex, N_face, P, K = ah.shape[1], fv.shape[1], q.shape[1], ix.shape[1]
+    tri = np.zeros(P, dtype=np.int64)
+    bary = np.zeros((3, P))
+    rc = lib().des_oracle_nearest_forward(
+        N_vertex, N_face, P, K, ahp, alp, al.size, fvp, vpp, qp, ixp,
+        tri.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+        bary.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    assert rc == 0
+    return tri, bary
+
+
+def nearest_mesh_points_backward(face_vertex_list, vertex_pos, query_pos, nearest_triangle_id,
+                                 grad_barycentric_coord):
+    """-> grad_query_pos 3 x P (FP64), upstream gradient from grad_barycentric_coord."""
+    fv, fvp = _typed(face_vertex_list, np.int32, ctypes.c_int32)
+    vp, vpp = _d(vertex_pos)
+    q, qp = _d(query_pos)
+    tr, trp = _typed(nearest_triangle_id, np.int64, ctypes.c_int64)
+    g, gp = _d(grad_barycentric_coord)
+    N_vertex, N_face, P = vp.shape[1], fv.shape[1], q.shape[1]
+    out = np.zeros((3, P))
+    rc = lib().des_oracle_nearest_backward(
+        N_vertex, N_face, P, fvp, vpp, qp, trp, gp,
+        out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    assert rc == 0
+    return out
+
+
+def extract_bitfield(bitfield, N_bit, i_start, i_end):
+    """bitfield N_slot x N int32 -> (i_end - i_start) x N_bit float64 of 0/1."""
+    b, bp = _typed(np.asarray(bitfield).view(np.uint32), np.uint32, ctypes.c_uint32)
+    N = b.shape[1]
+    out = np.zeros((max(i_end - i_start, 0), N_bit))
+    lib().des_oracle_extract_bitfield(bp, N, N_bit, i_start, i_end,
+                                      out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
     return out

@@ -195,3 +195,62 @@ def test_timing_events_need_both():
     L = _lib.load()
     assert L.des_set_backward_timing_events(ctypes.c_void_p(DUMMY), None) == -1
     assert L.des_set_backward_timing_events(None, None) == 0
+
+
+# ---- mesh utilities ----------------------------------------------------------------
+
+def test_header_declares_the_mesh_entry_points():
+    names = declared_symbols()
+    for n in ("get_nearest_mesh_points_forward", "get_nearest_mesh_points_backward",
+              "get_uv_of_triangle_forward", "get_uv_of_triangle_backward",
+              "extract_bitfield_forward"):
+        assert f"des_{n}" in names
+
+
+def _nearest_fwd(NV=10, NF=10, P=4, K=2, n_adj=30, ptr=DUMMY):
+    return _lib.load().des_get_nearest_mesh_points_forward(NV, NF, P, K, ptr, ptr, n_adj, ptr,
+                                                           ptr, ptr, ptr, ptr, ptr, None)
+
+
+@pytest.mark.parametrize("kw", [{"NV": -1}, {"NF": -1}, {"P": -1}, {"K": -1}, {"n_adj": -1},
+                                {"P": 1 << 31}, {"ptr": None}])
+def test_nearest_forward_rejects_invalid_arguments(kw):
+    assert _nearest_fwd(**kw) == -1
+    assert _lib.load().des_last_error().decode() != ""
+
+
+def test_mesh_entry_points_with_no_points_do_nothing():
+    L = _lib.load()
+    assert _nearest_fwd(P=0, ptr=None) == 0
+    assert L.des_get_nearest_mesh_points_backward(10, 10, 0, *([None] * 7)) == 0
+    assert L.des_get_uv_of_triangle_forward(0, None, None, None, None) == 0
+    assert L.des_get_uv_of_triangle_backward(0, None, None, None, None, None) == 0
+    assert L.des_get_uv_of_triangle_forward(-1, None, None, None, None) == -1
+    assert L.des_get_nearest_mesh_points_backward(10, 10, 5, DUMMY, DUMMY, DUMMY, None, DUMMY,
+                                                  DUMMY, None) == -1
+
+
+@pytest.mark.parametrize("args,ok", [
+    ((2, 100, 64, 0, 100), True),     # whole range, N_bit = 32 * N_slot
+    ((2, 100, 65, 0, 100), False),    # more bits than the slots hold
+    ((2, 100, 10, 5, 5), True),       # empty range
+    ((2, 100, 10, 6, 5), False),      # i_start > i_end
+    ((2, 100, 10, 0, 101), False),    # beyond N
+    ((2, 100, 10, -1, 3), False),
+    ((2, 100, 0, 0, 100), True),      # no bits
+])
+def test_extract_bitfield_validation(args, ok):
+    N_slot, N, N_bit, i0, i1 = args
+    L = _lib.load()
+    if ok and (i1 - i0) * N_bit > 0:
+        return  # valid and non-empty: would launch a kernel
+    rc = L.des_extract_bitfield_forward(DUMMY, N_slot, N, N_bit, i0, i1, DUMMY, None)
+    assert rc == (0 if ok else -1), _lib.load().des_last_error()
+
+
+def test_python_extract_bitfield_reports_invalid_ranges():
+    import torch
+    from diff_envshade_amd import _diff_envshade as de
+    # the checks run before any device work, so a CPU tensor is rejected first
+    with pytest.raises(RuntimeError, match="CUDA"):
+        de.extract_bitfield_forward(torch.zeros(2, 4, dtype=torch.int32), 8, 0, 4)

@@ -64,13 +64,25 @@ def test_cpu_tensors_are_rejected_like_torch_check():
 
 
 @pytest.mark.parametrize("name", ["blur_envmap_forward", "sample_envmap_forward",
-                                  "generate_camera_ray_forward", "get_nearest_mesh_points_forward",
-                                  "get_nearest_mesh_points_backward", "get_uv_of_triangle_forward",
-                                  "get_uv_of_triangle_backward", "mt_format_conversion_forward",
-                                  "extract_bitfield_forward"])
+                                  "generate_camera_ray_forward", "mt_format_conversion_forward"])
 def test_non_shader_bindings_are_outside_the_library(name):
     with pytest.raises(NotImplementedError):
         getattr(C, name)()
+
+
+def test_mesh_bindings_check_their_tensors():
+    # checked like the reference's CHECK_CUDA / CHECK_IS_* before any device work
+    q = torch.zeros(3, 4)
+    with pytest.raises(RuntimeError, match="CUDA"):
+        C.get_nearest_mesh_points_forward(None, None, None, None, q, None)
+    with pytest.raises(RuntimeError, match="CUDA"):
+        C.get_nearest_mesh_points_backward(None, None, q, None, None)
+    with pytest.raises(RuntimeError, match="CUDA"):
+        C.get_uv_of_triangle_forward(torch.zeros(4, 3, 3), torch.zeros(4, 3))
+    with pytest.raises(RuntimeError, match="CUDA"):
+        C.get_uv_of_triangle_backward(torch.zeros(4, 3, 3), torch.zeros(4, 3), torch.zeros(4, 3))
+    with pytest.raises(RuntimeError, match="tensor"):
+        C.extract_bitfield_forward(None, 8, 0, 1)
 
 
 def test_autograd_classes_exist():

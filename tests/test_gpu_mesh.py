@@ -1,0 +1,193 @@
+"""Mesh utilities on the GPU against the FP64 oracle (oracle/mesh_oracle.c).
+
+Both sides evaluate the reference's closest-point geometry (etc.cu:205-393) in
+FP64 with the same operations, each correctly rounded and none fused, so the
+outputs agree exactly: triangle ids bit for bit, and barycentrics and
+gradients as the float32 rounding of the oracle's FP64 values.  Exact
+agreement matters for the triangle id: a query whose closest point lies on an
+edge or vertex shared by several faces is equally near to all of them, and
+the reference's rule (first candidate at the smallest distance) then picks
+one of them by candidate order.
+"""
+import numpy as np
+import pytest
+import torch
+
+from diff_envshade_amd import _diff_envshade as de
+from diff_envshade_amd import workloads as WL
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+DEV = "cuda:0"
+
+
+def cuda(d):
+    return {k: torch.from_numpy(np.ascontiguousarray(v)).to(DEV) for k, v in d.items()}
+
+
+def f32(x):
+    return np.asarray(x, dtype=np.float64).astype(np.float32)
+
+
+def gpu_nearest(d):
+    t = cuda(d)
+    tri, bary = de.get_nearest_mesh_points_forward(t["adjacency_head"], t["adjacency_list"],
+                                                   t["face_vertex_list"], t["vertex_pos"],
+                                                   t["query_pos"], t["idxs"])
+    return tri.cpu().numpy(), bary.cpu().numpy()
+
+
+def oracle_nearest(d):
+    return O.nearest_mesh_points_forward(d["adjacency_head"], d["adjacency_list"],
+                                         d["face_vertex_list"], d["vertex_pos"], d["query_pos"],
+                                         d["idxs"])
+
+
+def check_nearest(d):
+    tri, bary = gpu_nearest(d)
+    rtri, rbary = oracle_nearest(d)
+    mism = np.nonzero(tri != rtri)[0]
+    assert mism.size == 0, f"{mism.size} triangle ids differ, first at {mism[:5]}"
+    assert np.array_equal(bary, f32(rbary)), np.abs(bary - rbary).max()
+    return tri, bary
+
+
+@pytest.mark.parametrize("subdiv,n,K", [(3, 5000, 8), (2, 777, 1), (2, 1000, 64), (2, 700, 80),
+                                        (1, 300, 42)])
+def test_nearest_forward_matches_oracle(subdiv, n, K):
+    # K = 64 fills one wave exactly, K = 80 and 42 x 6 faces take several
+    # 64-candidate rounds and vertex chunks
+    d = WL.mesh_queries(seed=subdiv * 100 + K, subdiv=subdiv, n=n, K=K)
+    tri, _ = check_nearest(d)
+    assert (tri >= 0).all()
+
+
+def test_nearest_forward_headline_workload():
+    # 70000 queries around a 20480-face surface, 8 nearest vertices each
+    d = WL.mesh_queries(seed=0, subdiv=5, n=70000, K=8)
+    check_nearest(d)
+
+
+def test_nearest_backward_matches_oracle():
+    d = WL.mesh_queries(seed=3, subdiv=3, n=4099, K=8)
+    tri, _ = oracle_nearest(d)
+    rng = np.random.default_rng(9)
+    g = rng.normal(size=(3, tri.size)).astype(np.float32)
+    tri = tri.copy()
+    tri[::97] = -1          # invalid ids give zero gradient
+    tri[1::101] = 10 ** 9
+    t = cuda(d)
+    out = de.get_nearest_mesh_points_backward(t["face_vertex_list"], t["vertex_pos"], t["query_pos"],
+                                              torch.from_numpy(tri).to(DEV),
+                                              torch.from_numpy(g).to(DEV)).cpu().numpy()
+    ref = O.nearest_mesh_points_backward(d["face_vertex_list"], d["vertex_pos"], d["query_pos"],
+                                         tri, g)
+    assert np.array_equal(out, f32(ref)), np.abs(out - ref).max()
+    assert np.all(out[:, ::97] == 0) and np.all(out[:, 1::101] == 0)
+    assert np.abs(out).max() > 0.1
+
+
+def _hand_mesh(lists, q, idxs):
+    v = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, -1], [1, 0, -1]], dtype=np.float32)
+    f = np.array([[0, 1, 2], [0, 1, 2], [3, 4, 0]], dtype=np.int32)
+    count = np.array([len(l) for l in lists], dtype=np.int32)
+    return {
+        "adjacency_head": np.stack([count, np.cumsum(count).astype(np.int32)]),
+        "adjacency_list": np.array([x for l in lists for x in l] or [0], dtype=np.int32),
+        "face_vertex_list": np.ascontiguousarray(f.T),
+        "vertex_pos": np.ascontiguousarray(v.T),
+        "query_pos": np.asarray(q, dtype=np.float32).reshape(3, -1),
+        "idxs": np.asarray(idxs, dtype=np.int64),
+    }
+
+
+@pytest.mark.parametrize("lists,idxs,want", [
+    ([[1, 0, 2], [0], [], [2], []], [[0]], 1),    # equal distance: first candidate
+    ([[0, 1, 2], [1], [], [2], []], [[0]], 0),
+    ([[1, 0, 2], [0], [], [2], []], [[1, 0]], 0), # vertex order first, repeats change nothing
+    ([[2, 1], [0], [], [2], []], [[0]], 1),       # a strictly nearer later face wins
+    ([[1, 0, 2], [0], [], [2], []], [[-1, 7]], -1),  # no valid vertex
+    ([[1, 0, 2], [0], [], [2], []], [[2, 2]], -1),   # a vertex with no faces
+])
+def test_nearest_forward_tie_break_and_no_candidate(lists, idxs, want):
+    d = _hand_mesh(lists, [0.2, 0.3, 0.5], idxs)
+    tri, bary = check_nearest(d)
+    assert tri[0] == want
+    if want < 0:
+        assert np.all(bary == 0)
+
+
+def test_nearest_forward_without_neighbours_and_points():
+    d = WL.mesh_queries(seed=1, subdiv=1, n=100, K=4)
+    d["idxs"] = np.zeros((100, 0), dtype=np.int64)
+    tri, bary = gpu_nearest(d)
+    assert np.all(tri == -1) and np.all(bary == 0)
+    d = WL.mesh_queries(seed=1, subdiv=1, n=0, K=4)
+    tri, bary = gpu_nearest(d)
+    assert tri.shape == (0,) and bary.shape == (3, 0)
+
+
+def test_uv_of_triangle_matches_oracle():
+    rng = np.random.default_rng(11)
+    P = 3001
+    tri = rng.normal(size=(P, 3, 3)).astype(np.float32)
+    q = (rng.normal(size=(P, 3)) * 2).astype(np.float32)
+    g = rng.normal(size=(P, 3)).astype(np.float32)
+    tt, qt, gt = (torch.from_numpy(x).to(DEV) for x in (tri, q, g))
+    bary = de.get_uv_of_triangle_forward(tt, qt).cpu().numpy()
+    grad = de.get_uv_of_triangle_backward(tt, qt, gt).cpu().numpy()
+    assert np.array_equal(bary, f32(O.uv_of_triangle_forward(tri, q)))
+    assert np.array_equal(grad, f32(O.uv_of_triangle_backward(tri, q, g)))
+
+
+@pytest.mark.parametrize("N_slot,N,N_bit,i0,i1", [
+    (16, 70000, 512, 0, 70000),   # the 16 x 32 env map's words, every point
+    (3, 1000, 77, 13, 900),       # ragged last word, offset range
+    (1, 50, 1, 0, 50),
+    (2, 64, 33, 63, 64),
+    (4, 200, 100, 0, 0),          # empty range
+    (2, 300, 64, 1, 299),
+])
+def test_extract_bitfield_matches_oracle(N_slot, N, N_bit, i0, i1):
+    rng = np.random.default_rng(N_bit)
+    words = rng.integers(0, 2 ** 32, size=(N_slot, N), dtype=np.uint64).astype(np.uint32).view(np.int32)
+    out = de.extract_bitfield_forward(torch.from_numpy(words).to(DEV), N_bit, i0, i1).cpu().numpy()
+    assert out.dtype == np.float32 and out.shape == (i1 - i0, N_bit)
+    assert np.array_equal(out, O.extract_bitfield(words, N_bit, i0, i1).astype(np.float32))
+
+
+def test_extract_bitfield_rejects_invalid_ranges():
+    words = torch.zeros(2, 10, dtype=torch.int32, device=DEV)
+    with pytest.raises(RuntimeError, match="i_start"):
+        de.extract_bitfield_forward(words, 8, 5, 11)
+    with pytest.raises(RuntimeError, match="slots"):
+        de.extract_bitfield_forward(words, 65, 0, 10)
+
+
+def test_nearest_points_feed_the_visibility_shader():
+    # the mesh use case: per-vertex visibility words, the nearest face and its
+    # barycentrics from the GPU search, shaded by the visibility shader
+    from _helpers import TOL, relerr
+    d = WL.mesh_queries(seed=4, subdiv=3, n=3000, K=8)
+    tri, bary = check_nearest(d)
+    rng = np.random.default_rng(4)
+    H, W = 8, 16
+    n = tri.size
+    p = WL.make_points(rng, n)
+    p["envmap"] = WL.make_envmap(rng, H, W, 1.0)
+    p["light2obj_rotmat"] = WL.random_rotation(rng)
+    nv = d["vertex_pos"].shape[1]
+    words = rng.integers(0, 2 ** 32, size=((H * W + 31) // 32, nv), dtype=np.uint64)
+    p["visibility"] = words.astype(np.uint32).view(np.int32)
+    p["face_vertex_list"] = d["face_vertex_list"]
+    p["nearest_triangle_id"] = tri
+    p["barycentric_coord"] = bary
+    t = cuda(p)
+    sh, ds = de.bruteforce_specularvisibility_shader_forward2(
+        t["normal"], t["view_dir"], t["albedo"], t["specular_albedo"], t["roughness"], t["envmap"],
+        t["light2obj_rotmat"], t["face_vertex_list"], t["nearest_triangle_id"],
+        t["barycentric_coord"], t["visibility"], True)
+    rsh, rds = O.forward("vis", p, True, True)
+    assert relerr(sh.cpu().numpy(), rsh) < TOL
+    assert relerr(ds.cpu().numpy(), rds) < TOL
