@@ -42,6 +42,19 @@ HEADLINE_TRAFFIC = {"points": 70000, "texels": 512, "fetch_kib": 52914.9, "fetch
                     "write_kib": 23761.0, "write_factor": 1.0}
 
 
+# The same two counters on the mesh workloads (the forward of nearest_mesh,
+# the decode of bitfield), 100-launch averages, same scaling; commands and
+# figures in profiles/nearest_mesh_70k_ico5_k8.txt and
+# profiles/bitfield_70k_512bit.txt.
+NEAREST_TRAFFIC = {"queries": 70000, "fetch_kib": 5771.07, "write_kib": 1367.19}
+BITFIELD_TRAFFIC = {"points": 70000, "fetch_kib": 2752.04, "write_kib": 140000.0}
+
+
+def _scaled_bytes(t):
+    f = HEADLINE_TRAFFIC
+    return (t["fetch_kib"] * f["fetch_factor"] + t["write_kib"] * f["write_factor"]) * 1024.0
+
+
 def _traffic(P, T):
     t = HEADLINE_TRAFFIC
     if (P, T) != (t["points"], t["texels"]):
@@ -396,7 +409,8 @@ def bench_nearest_mesh(args):
                      "achieved": achieved / 1e12, "peak": PEAK_FP64_FLOPS / 1e12,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_FLOPS,
                      "flop_per_candidate": FLOP_NEAREST_PER_CANDIDATE,
-                     "traffic": None,
+                     "traffic": (_scaled_bytes(NEAREST_TRAFFIC)
+                                 if n == NEAREST_TRAFFIC["queries"] else None),
                      "hbm_compulsory_bytes": fwd_bytes,
                      "hbm_frac": fwd_bytes / (fwd_ms * 1e-3) / PEAK_HBM_BYTES},
     }
@@ -469,7 +483,9 @@ def bench_bitfield(args):
         "kernels_us": {"bitfield": k_ms * 1e3},
         "roofline": {"bound": "hbm", "kernel": "bitfield_kernel", "achieved": achieved / 1e9,
                      "peak": PEAK_HBM_BYTES / 1e9, "unit": "GB/s",
-                     "frac": achieved / PEAK_HBM_BYTES, "traffic": None,
+                     "frac": achieved / PEAK_HBM_BYTES,
+                     "traffic": (_scaled_bytes(BITFIELD_TRAFFIC)
+                                 if n == BITFIELD_TRAFFIC["points"] else None),
                      "bytes_per_launch": nbytes},
     }
     if not (args.no_accuracy and args.no_cpu_baseline):

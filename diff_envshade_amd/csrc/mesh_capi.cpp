@@ -19,6 +19,8 @@ using des_capi::fail;
 using des_capi::ok;
 
 constexpr int64_t kMaxPoints = int64_t(1) << 31;  // the reference indexes points with int
+// adjacency entries: a group of 16 lanes sums 16 face counts in 32 bits
+constexpr int64_t kMaxAdjacency = int64_t(1) << 26;
 
 int launch_err(hipError_t e, const char* what) {
   return fail(DES_ERR_LAUNCH, "%s: %s", what, hipGetErrorString(e));
@@ -40,7 +42,8 @@ DES_API int des_get_nearest_mesh_points_forward(
     int64_t* nearest_triangle_id, float* barycentric_coord, void* stream) {
   if (int rc = check_mesh(N_vertex, N_face, P)) return rc;
   if (K < 0) return fail(DES_ERR_INVALID_ARGUMENT, "K must be >= 0");
-  if (N_adjacency < 0) return fail(DES_ERR_INVALID_ARGUMENT, "N_adjacency must be >= 0");
+  if (N_adjacency < 0 || N_adjacency >= kMaxAdjacency)
+    return fail(DES_ERR_INVALID_ARGUMENT, "N_adjacency must be in [0, 2^26)");
   if (P == 0) return ok();
   if (!query_pos || !nearest_triangle_id || !barycentric_coord || (K > 0 && !idxs))
     return fail(DES_ERR_INVALID_ARGUMENT, "query_pos, idxs and outputs must not be NULL");
@@ -132,7 +135,7 @@ DES_API int des_extract_bitfield_forward(const int32_t* bitfield, int32_t N_slot
   const int64_t nb = i_end - i_start;
   if (nb == 0 || N_bit == 0) return ok();
   if (!bitfield || !output) return fail(DES_ERR_INVALID_ARGUMENT, "bitfield and output must not be NULL");
-  if (nb * ((N_bit + 31) / 32) >= (int64_t(1) << 39))
+  if (nb >= (int64_t(1) << 37))  // 64 points per block of the launch grid
     return fail(DES_ERR_INVALID_ARGUMENT, "bitfield range too large");
   const hipError_t e = des::launch_bitfield(reinterpret_cast<const uint32_t*>(bitfield), N, N_bit,
                                             i_start, nb, output, static_cast<hipStream_t>(stream));

@@ -2,7 +2,8 @@
  * diff_envshade.h -- C ABI of the MI355X (gfx950) brute-force environment-light
  * shader.  Drop-in boundary for the ten shader bindings of zjumsj/diff_envshade
  * (bindings.cpp:6-19, prototypes gsface_shader.h:3-118, host wrappers
- * bruteforce_shader.cu:2687-3453).
+ * bruteforce_shader.cu:2687-3453) and for the five mesh utilities that produce
+ * its mesh inputs (bindings.cpp:27-30, 33; etc.cu), declared at the end.
  *
  * Conventions (all entry points):
  *   - every pointer is a device pointer to contiguous row-major data on the

@@ -25,8 +25,13 @@
 
 typedef struct { double x, y, z; } m3;
 
+/* The FP64 evaluation is a fixed sequence of IEEE operations -- adds,
+ * multiplies, fused multiply-adds where written as fma(), one correctly
+ * rounded division per face -- with no other contraction (built with
+ * -ffp-contract=off), so every result is a function of the inputs alone. */
 static inline m3 msub(m3 a, m3 b) { m3 c = {a.x - b.x, a.y - b.y, a.z - b.z}; return c; }
-static inline double mdot(m3 a, m3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static inline double mdot(m3 a, m3 b) { return fma(a.z, b.z, fma(a.y, b.y, a.x * b.x)); }
+/* etc.cu:63-67: a NaN passes through */
 static inline double mclamp01(double v) { return v < 0.0 ? 0.0 : (v > 1.0 ? 1.0 : v); }
 
 /* Region of the closest point (etc.cu:316-354): 0 interior, 1 edge x0-x1 (s),
@@ -46,9 +51,12 @@ static int region(double a, double b, double c, double d, double e, double det, 
   return 3;
 }
 
+/* Per face: the quantities of etc.cu:310-320 and the one quotient the region
+ * needs, taken as num * (1 / den): interior (S, T) / det, edge x0-x1 -d / a,
+ * edge x0-x2 -e / c, edge x1-x2 (c + e - b - d) / (a - 2b + c). */
 typedef struct {
   m3 e0, e1;
-  double a, b, c, d, e, det, s, t;
+  double a, b, c, d, e, det, S, T, inv, u;
   int type;
 } tri_geo;
 
@@ -62,34 +70,35 @@ static tri_geo geometry(m3 x0, m3 x1, m3 x2, m3 q) {
   g.c = mdot(g.e1, g.e1);
   g.d = mdot(g.e0, v0);
   g.e = mdot(g.e1, v0);
-  g.det = g.a * g.c - g.b * g.b;
-  g.s = g.b * g.e - g.c * g.d;
-  g.t = g.b * g.d - g.a * g.e;
-  g.type = region(g.a, g.b, g.c, g.d, g.e, g.det, g.s, g.t);
+  g.det = fma(g.a, g.c, -(g.b * g.b));
+  g.S = fma(g.b, g.e, -(g.c * g.d));
+  g.T = fma(g.b, g.d, -(g.a * g.e));
+  g.type = region(g.a, g.b, g.c, g.d, g.e, g.det, g.S, g.T);
+  double num, den;
+  switch (g.type) {
+    case 0: num = g.S; den = g.det; break;
+    case 1: num = -g.d; den = g.a; break;
+    case 2: num = -g.e; den = g.c; break;
+    default: num = ((g.c + g.e) - g.b) - g.d; den = (g.a - 2.0 * g.b) + g.c; break;
+  }
+  g.inv = 1.0 / den;
+  g.u = num * g.inv;
   return g;
 }
 
 /* etc.cu:356-392: (s, t) of the closest point x0 + s e0 + t e1. */
+static void st_of(const tri_geo* g, double* os, double* ot) {
+  switch (g->type) {
+    case 0: *os = g->u; *ot = g->T * g->inv; break;
+    case 1: *os = mclamp01(g->u); *ot = 0.0; break;
+    case 2: *os = 0.0; *ot = mclamp01(g->u); break;
+    default: *os = mclamp01(g->u); *ot = 1.0 - *os; break;
+  }
+}
+
 static void closest_st(m3 x0, m3 x1, m3 x2, m3 q, double* os, double* ot) {
   tri_geo g = geometry(x0, x1, x2, q);
-  double s = g.s, t = g.t;
-  if (g.type == 0) {
-    double inv = 1.0 / g.det;
-    s *= inv;
-    t *= inv;
-  } else if (g.type == 1) {
-    s = mclamp01(-g.d / g.a);
-    t = 0.0;
-  } else if (g.type == 2) {
-    s = 0.0;
-    t = mclamp01(-g.e / g.c);
-  } else {
-    double numer = g.c + g.e - g.b - g.d, denom = g.a - 2.0 * g.b + g.c;
-    s = mclamp01(numer / denom);
-    t = 1.0 - s;
-  }
-  *os = s;
-  *ot = t;
+  st_of(&g, os, ot);
 }
 
 /* etc.cu:205-300: grad of the query point from grad_s, grad_t (the clamps pass
@@ -98,34 +107,24 @@ static m3 closest_st_vjp(m3 x0, m3 x1, m3 x2, m3 q, double gs, double gt) {
   tri_geo g = geometry(x0, x1, x2, q);
   m3 r = {0.0, 0.0, 0.0};
   if (g.type == 0) {
-    double inv = 1.0 / g.det;
-    double u = (gs * g.c - gt * g.b) * inv, w = (gt * g.a - gs * g.b) * inv;
-    r.x = g.e0.x * u + g.e1.x * w;
-    r.y = g.e0.y * u + g.e1.y * w;
-    r.z = g.e0.z * u + g.e1.z * w;
-  } else if (g.type == 1) {
-    double v = -g.d / g.a;
-    if (v >= 0.0 && v <= 1.0) {
-      r.x = g.e0.x * gs / g.a;
-      r.y = g.e0.y * gs / g.a;
-      r.z = g.e0.z * gs / g.a;
-    }
+    double u = fma(gs, g.c, -(gt * g.b)) * g.inv, w = fma(gt, g.a, -(gs * g.b)) * g.inv;
+    r.x = fma(g.e1.x, w, g.e0.x * u);
+    r.y = fma(g.e1.y, w, g.e0.y * u);
+    r.z = fma(g.e1.z, w, g.e0.z * u);
+    return r;
+  }
+  if (!(g.u >= 0.0 && g.u <= 1.0)) return r;
+  if (g.type == 1) {
+    double k = gs * g.inv;
+    r.x = g.e0.x * k; r.y = g.e0.y * k; r.z = g.e0.z * k;
   } else if (g.type == 2) {
-    double v = -g.e / g.c;
-    if (v >= 0.0 && v <= 1.0) {
-      r.x = g.e1.x * gt / g.c;
-      r.y = g.e1.y * gt / g.c;
-      r.z = g.e1.z * gt / g.c;
-    }
+    double k = gt * g.inv;
+    r.x = g.e1.x * k; r.y = g.e1.y * k; r.z = g.e1.z * k;
   } else {
-    double numer = g.c + g.e - g.b - g.d, denom = g.a - 2.0 * g.b + g.c;
-    double v = numer / denom;
-    if (v >= 0.0 && v <= 1.0) {
-      double k = (gs - gt) / denom;
-      r.x = (g.e0.x - g.e1.x) * k;
-      r.y = (g.e0.y - g.e1.y) * k;
-      r.z = (g.e0.z - g.e1.z) * k;
-    }
+    double k = (gs - gt) * g.inv;
+    r.x = (g.e0.x - g.e1.x) * k;
+    r.y = (g.e0.y - g.e1.y) * k;
+    r.z = (g.e0.z - g.e1.z) * k;
   }
   return r;
 }
@@ -223,11 +222,11 @@ int des_oracle_nearest_forward(int N_vertex, int64_t N_face, int64_t P, int K,
         seen[nseen++] = f;
         m3 x[3];
         if (!face_vertices(fvl, N_face, N_vertex, f, vpos, x)) continue;
+        tri_geo g = geometry(x[0], x[1], x[2], qp);
         double s, t;
-        closest_st(x[0], x[1], x[2], qp, &s, &t);
-        m3 cp = {x[0].x + s * (x[1].x - x[0].x) + t * (x[2].x - x[0].x),
-                 x[0].y + s * (x[1].y - x[0].y) + t * (x[2].y - x[0].y),
-                 x[0].z + s * (x[1].z - x[0].z) + t * (x[2].z - x[0].z)};
+        st_of(&g, &s, &t);
+        m3 cp = {fma(t, g.e1.x, fma(s, g.e0.x, x[0].x)), fma(t, g.e1.y, fma(s, g.e0.y, x[0].y)),
+                 fma(t, g.e1.z, fma(s, g.e0.z, x[0].z))};
         m3 df = msub(cp, qp);
         double d2 = mdot(df, df);
         if (d2 < best) {

@@ -213,7 +213,7 @@ def _nearest_fwd(NV=10, NF=10, P=4, K=2, n_adj=30, ptr=DUMMY):
 
 
 @pytest.mark.parametrize("kw", [{"NV": -1}, {"NF": -1}, {"P": -1}, {"K": -1}, {"n_adj": -1},
-                                {"P": 1 << 31}, {"ptr": None}])
+                                {"n_adj": 1 << 26}, {"P": 1 << 31}, {"ptr": None}])
 def test_nearest_forward_rejects_invalid_arguments(kw):
     assert _nearest_fwd(**kw) == -1
     assert _lib.load().des_last_error().decode() != ""
