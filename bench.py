@@ -91,14 +91,16 @@ def main() -> int:
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--workload", default="dropin",
                     choices=["dropin", "batched", "relight", "serving", "copy", "nearest_mesh",
-                             "bitfield"],
+                             "bitfield", "mesh_vis"],
                     help="dropin (default, the headline); batched: 48 uneven point sets in one "
                          "persistent launch; relight: 64 env maps x 262,144 points on MFMA; "
                          "serving: Poisson arrivals through the continuous-batching server; "
                          "copy: a plain 256 MiB device copy, the calibration kernel for the "
                          "HBM byte counters; nearest_mesh: nearest mesh points forward+backward "
                          "for 70,000 queries; bitfield: decoding the headline's 512-bit "
-                         "visibility words to floats")
+                         "visibility words to floats; mesh_vis: the headline's shading with "
+                         "per-vertex visibility on a 20,480-face surface, through each point's "
+                         "nearest face")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the shader has no "
@@ -107,7 +109,8 @@ def main() -> int:
     if args.workload != "dropin":
         line = {"batched": bench_batched, "relight": bench_relight,
                 "serving": bench_serving, "copy": bench_copy,
-                "nearest_mesh": bench_nearest_mesh, "bitfield": bench_bitfield}[args.workload](args)
+                "nearest_mesh": bench_nearest_mesh, "bitfield": bench_bitfield,
+                "mesh_vis": bench_mesh_vis}[args.workload](args)
         print(json.dumps(line), flush=True)
         return 0
 
@@ -506,6 +509,102 @@ def bench_bitfield(args):
                                               "scalar C oracle), median of "
                                               f"{len(timed)} runs after 1 warm-up",
                                     "seconds_per_step": med}
+    return line
+
+
+def bench_mesh_vis(args):
+    """Vertex-indexed visibility at mesh scale: 70,000 points around a closed
+    20,480-face surface (10,242 vertices), each shaded through its nearest face
+    and barycentrics (from get_nearest_mesh_points_forward, before timing) with
+    per-vertex 512-bit visibility, outward normals, the headline's 16x32 env map
+    and materials; a step is one forward + one backward through
+    bruteforce_specularvisibility_shader_{forward2,backward2}."""
+    import numpy as np
+    import torch
+
+    from diff_envshade_amd import _diff_envshade as C
+    from diff_envshade_amd import _lib
+    from diff_envshade_amd import workloads as WL
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    H, W = 16, 32
+    d = WL.mesh_shading(seed=0, n=args.points, H=H, W=W)
+    m = {k: torch.from_numpy(v).to(dev) for k, v in d.pop("mesh").items()}
+    tri, bary = C.get_nearest_mesh_points_forward(m["adjacency_head"], m["adjacency_list"],
+                                                  m["face_vertex_list"], m["vertex_pos"],
+                                                  m["query_pos"], m["idxs"])
+    torch.cuda.synchronize()
+    d["nearest_triangle_id"] = tri.cpu().numpy()
+    d["barycentric_coord"] = bary.cpu().numpy()
+    P = d["normal"].shape[0]
+    t = {k: torch.from_numpy(np.ascontiguousarray(v)).to(dev) for k, v in d.items()}
+    mat = [t["normal"], t["view_dir"], t["albedo"], t["specular_albedo"], t["roughness"],
+           t["envmap"], t["light2obj_rotmat"]]
+    vis = [t["face_vertex_list"], t["nearest_triangle_id"], t["barycentric_coord"],
+           t["visibility"]]
+
+    def step(kev=None):
+        sh, ds = C.bruteforce_specularvisibility_shader_forward2(*mat, *vis, True)
+        if kev is not None:
+            _lib.set_backward_timing_events(kev[0], kev[1])
+        g = C.bruteforce_specularvisibility_shader_backward2(
+            t["grad_shading"], t["grad_diffuse_shading"], *mat, *vis, True)
+        return sh, ds, g
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    K = args.steps
+    t0 = time.perf_counter()
+    for _ in range(K):
+        out = step()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / K
+    ev = lambda: torch.cuda.Event(enable_timing=True)  # noqa: E731
+    kern = [(ev(), ev()) for _ in range(min(K, 100))]
+    for a, b in kern:  # torch creates the HIP events on first record
+        a.record()
+        b.record()
+    torch.cuda.synchronize()
+    for a, b in kern:
+        step((a, b))
+    torch.cuda.synchronize()
+    bwd_ms = sum(a.elapsed_time(b) for a, b in kern) / len(kern)
+    T = H * W
+    achieved = P * T * FLOP_BWD_PER_PAIR / (bwd_ms * 1e-3)
+    print(f"[bench] mesh_vis: {ms:.4f} ms/step, backward kernel {bwd_ms:.4f} ms", file=sys.stderr,
+          flush=True)
+    line = {
+        "metric": "point·envmap pairs/s fwd+bwd, per-vertex visibility through nearest faces",
+        "value": P / (ms * 1e-3), "unit": UNIT, "n_gpus": 1, "steps": K, "warmup": args.warmup,
+        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32", "data": "synthetic",
+        "config": {"workload": f"{P} points around a 20,480-face surface (10,242 vertices), "
+                               f"one {H}x{W} env map, 512-bit per-vertex visibility, nearest "
+                               "face + barycentrics per point, forward+backward",
+                   "points": P, "vertices": int(d["visibility"].shape[1]),
+                   "faces": int(d["face_vertex_list"].shape[1])},
+        "kernels_ms": {"backward_main": bwd_ms},
+        "roofline": {"bound": "mfma", "pipe": "fp32 vector ALU (no MFMA on this path)",
+                     "kernel": "shade_backward_kernel<vis>", "achieved": achieved / 1e12,
+                     "peak": PEAK_FP32_FLOPS / 1e12, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP32_FLOPS, "flop_per_pair": FLOP_BWD_PER_PAIR,
+                     "traffic": None},
+    }
+    if not args.no_accuracy:
+        O = _oracle()
+        sh, ds, g = out
+        rsh, rds = O.forward("vis", d)
+        rg = O.backward("vis", d, d["grad_shading"], d["grad_diffuse_shading"])
+        gn, gv, ga, gs, gr, gE = [x.cpu().numpy() for x in g]
+        line["err_vs_fp64"] = {
+            "shading": _errors(sh.cpu().numpy(), rsh),
+            "diffuse_shading": _errors(ds.cpu().numpy(), rds),
+            "grad_normal": _errors(gn, rg["normal"]), "grad_view_dir": _errors(gv, rg["view_dir"]),
+            "grad_albedo": _errors(ga, rg["albedo"]),
+            "grad_specular_albedo": _errors(gs, rg["specular_albedo"]),
+            "grad_roughness": _errors(gr, rg["roughness"]), "grad_envmap": _errors(gE, rg["envmap"]),
+        }
     return line
 
 

@@ -293,3 +293,27 @@ def mesh_queries(seed: int = 0, subdiv: int = 5, n: int = 70000, K: int = 8,
         "query_pos": np.ascontiguousarray(q.T),
         "idxs": idxs,
     }
+
+
+def mesh_shading(seed: int = 0, subdiv: int = 5, n: int = 70000, K: int = 8, H: int = 16,
+                 W: int = 32) -> dict:
+    """Vertex-indexed visibility at mesh scale: the points and surface of
+    mesh_queries, outward normals (the point's direction from the centre), the
+    drop-in headline's materials, 16x32 env map with 3 bright texels and
+    upstream gradients, and per-vertex 512-bit visibility (Bernoulli(0.7)
+    masked to the vertex's outward hemisphere, N_slot x N_vertex).  The
+    nearest face and barycentrics of each point are left to the caller (the
+    nearest-mesh-point search); the mesh arrays are under "mesh"."""
+    m = mesh_queries(seed=seed, subdiv=subdiv, n=n, K=K)
+    rng = np.random.default_rng(seed + 1)
+    d = make_points(rng, n)
+    q = m["query_pos"].T.astype(np.float64)
+    d["normal"] = (q / np.linalg.norm(q, axis=1, keepdims=True)).astype(np.float32)
+    d["envmap"] = make_envmap(rng, H, W, 1.0, n_bright=3)
+    d["light2obj_rotmat"] = np.eye(3, dtype=np.float32)
+    vp = m["vertex_pos"].T.astype(np.float64)
+    d["visibility"] = make_visibility(rng, vp / np.linalg.norm(vp, axis=1, keepdims=True), H, W)
+    d["face_vertex_list"] = m["face_vertex_list"]
+    d.update(upstream_grads(rng, n))
+    d["mesh"] = m
+    return d
