@@ -91,7 +91,7 @@ def main() -> int:
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--workload", default="dropin",
                     choices=["dropin", "batched", "relight", "serving", "copy", "nearest_mesh",
-                             "bitfield", "mesh_vis"],
+                             "bitfield", "mesh_vis", "envmap_sample", "envmap_blur"],
                     help="dropin (default, the headline); batched: 48 uneven point sets in one "
                          "persistent launch; relight: 64 env maps x 262,144 points on MFMA; "
                          "serving: Poisson arrivals through the continuous-batching server; "
@@ -100,7 +100,9 @@ def main() -> int:
                          "for 70,000 queries; bitfield: decoding the headline's 512-bit "
                          "visibility words to floats; mesh_vis: the headline's shading with "
                          "per-vertex visibility on a 20,480-face surface, through each point's "
-                         "nearest face")
+                         "nearest face; envmap_sample: 1,000,000 bilinear lookups in a 128x256 "
+                         "map; envmap_blur: GGX blur of a 128x256 map to 32x64 with 1024 "
+                         "samples per texel")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the shader has no "
@@ -110,7 +112,8 @@ def main() -> int:
         line = {"batched": bench_batched, "relight": bench_relight,
                 "serving": bench_serving, "copy": bench_copy,
                 "nearest_mesh": bench_nearest_mesh, "bitfield": bench_bitfield,
-                "mesh_vis": bench_mesh_vis}[args.workload](args)
+                "mesh_vis": bench_mesh_vis, "envmap_sample": bench_envmap_sample,
+                "envmap_blur": bench_envmap_blur}[args.workload](args)
         print(json.dumps(line), flush=True)
         return 0
 
@@ -605,6 +608,139 @@ def bench_mesh_vis(args):
             "grad_specular_albedo": _errors(gs, rg["specular_albedo"]),
             "grad_roughness": _errors(gr, rg["roughness"]), "grad_envmap": _errors(gE, rg["envmap"]),
         }
+    return line
+
+
+# FP64 operations per blur sample (3 channels), counted on etc.cu:588-687 with
+# add/sub/mul/div/sqrt and each of sin, cos, atan2, asin, floor = 1: the two
+# uniforms 4, the GGX half-vector 16, its direction in the texel's frame and
+# normalisation 24, the lookup position 15, the three bilinear blends and sums 30.
+FLOP_BLUR_PER_SAMPLE = 89
+
+
+def bench_envmap_sample(args):
+    """sample_envmap_forward: 1,000,000 random query directions, one 3 x 128 x 256
+    lognormal map."""
+    import numpy as np
+    import torch
+
+    from diff_envshade_amd import _diff_envshade as C
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    P, H, W = 1_000_000, 128, 256
+    rng = np.random.default_rng(0)
+    env_np = np.exp(rng.normal(size=(3, H, W))).astype(np.float32)
+    q_np = rng.normal(size=(3, P)).astype(np.float32)
+    env, q = torch.from_numpy(env_np).to(dev), torch.from_numpy(q_np).to(dev)
+
+    def call():
+        return C.sample_envmap_forward(q, env)
+
+    for _ in range(args.warmup):
+        call()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        out = call()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    k_ms = _queued_avg_ms(call, min(args.steps, 200))
+    nbytes = q_np.nbytes + 3 * P * 4 + env_np.nbytes
+    achieved = nbytes / (k_ms * 1e-3)
+    print(f"[bench] envmap_sample: {ms:.4f} ms/call, kernel {k_ms * 1e3:.2f} us", file=sys.stderr,
+          flush=True)
+    line = {
+        "metric": "env-map lookups/s (sample_envmap_forward)", "value": P / (ms * 1e-3),
+        "unit": "lookups/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"{P} random query directions, one 3x{H}x{W} map, bilinear "
+                               "lookup with the seam wrapped"},
+        "kernels_us": {"sample_envmap": k_ms * 1e3},
+        "roofline": {"bound": "hbm", "kernel": "sample_envmap_kernel", "achieved": achieved / 1e9,
+                     "peak": PEAK_HBM_BYTES / 1e9, "unit": "GB/s",
+                     "frac": achieved / PEAK_HBM_BYTES, "traffic": None,
+                     "bytes_per_launch": nbytes},
+    }
+    if not (args.no_accuracy and args.no_cpu_baseline):
+        O = _oracle()
+        runs = []
+        for _ in range(1 if args.no_cpu_baseline else 4):
+            a = time.perf_counter()
+            ref = O.sample_envmap(q_np, env_np)
+            runs.append(time.perf_counter() - a)
+        line["err_vs_fp64"] = _errors(out.cpu().numpy(), ref)
+        if not args.no_cpu_baseline:
+            timed = sorted(runs[1:])
+            med = timed[len(timed) // 2]
+            line["cpu_baseline"] = {"value": P / med, "unit": "lookups/s",
+                                    "cores": O.max_threads(), "kind": "port",
+                                    "sample": f"the full workload ({P} lookups, FP64 oracle, "
+                                              f"OpenMP), median of {len(timed)} runs after 1 "
+                                              "warm-up", "seconds_per_step": med}
+    return line
+
+
+def bench_envmap_blur(args):
+    """blur_envmap_forward: a 3 x 128 x 256 lognormal map blurred onto 32 x 64
+    texels with 1024 GGX samples each (alpha 0.3)."""
+    import numpy as np
+    import torch
+
+    from diff_envshade_amd import _diff_envshade as C
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    H, W, tH, tW, N, alpha = 128, 256, 32, 64, 1024, 0.3
+    rng = np.random.default_rng(0)
+    env_np = np.exp(rng.normal(size=(3, H, W))).astype(np.float32)
+    env = torch.from_numpy(env_np).to(dev)
+
+    def call():
+        return C.blur_envmap_forward(env, tH, tW, N, alpha)
+
+    for _ in range(args.warmup):
+        call()
+    torch.cuda.synchronize()
+    steps = max(1, min(args.steps, 100))
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        out = call()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    k_ms = _queued_avg_ms(call, min(steps, 50))
+    samples = tH * tW * N
+    achieved = samples * FLOP_BLUR_PER_SAMPLE / (k_ms * 1e-3)
+    print(f"[bench] envmap_blur: {ms:.4f} ms/call, kernel {k_ms * 1e3:.2f} us", file=sys.stderr,
+          flush=True)
+    line = {
+        "metric": "env-map blur samples/s (blur_envmap_forward)", "value": samples / (ms * 1e-3),
+        "unit": "samples/s", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
+        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"3x{H}x{W} map -> 3x{tH}x{tW}, {N} GGX samples per texel, "
+                               f"alpha {alpha}"},
+        "kernels_us": {"blur_envmap": k_ms * 1e3},
+        "roofline": {"bound": "fp64", "kernel": "blur_envmap_kernel<3>",
+                     "achieved": achieved / 1e12, "peak": PEAK_FP64_FLOPS / 1e12,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_FLOPS,
+                     "flop_per_sample": FLOP_BLUR_PER_SAMPLE, "traffic": None},
+    }
+    if not (args.no_accuracy and args.no_cpu_baseline):
+        O = _oracle()
+        runs = []
+        for _ in range(1 if args.no_cpu_baseline else 3):
+            a = time.perf_counter()
+            ref = O.blur_envmap(env_np, tH, tW, N, alpha)
+            runs.append(time.perf_counter() - a)
+        line["err_vs_fp64"] = _errors(out.cpu().numpy(), ref)
+        if not args.no_cpu_baseline:
+            timed = sorted(runs[1:])
+            med = timed[len(timed) // 2]
+            line["cpu_baseline"] = {"value": samples / med, "unit": "samples/s",
+                                    "cores": O.max_threads(), "kind": "port",
+                                    "sample": f"the full workload ({samples} samples, FP64 "
+                                              f"oracle, OpenMP), median of {len(timed)} runs "
+                                              "after 1 warm-up", "seconds_per_step": med}
     return line
 
 

@@ -16,9 +16,10 @@ reproducible.
 
 The mesh utilities that produce the shader's mesh inputs (nearest mesh
 points, closest point on a given triangle, visibility bitfield decoding;
-bindings.cpp:27-33, etc.cu) follow the same conventions.  The other etc.cu
-bindings (camera rays, env-map blur / sampling, rotation conversion) are
-outside this library and raise NotImplementedError.
+bindings.cpp:27-33, etc.cu) and the env-map sampling and blur beside it
+(bindings.cpp:20, 22) follow the same conventions.  The other etc.cu bindings
+(camera rays, rotation conversion) are outside this library and raise
+NotImplementedError.
 """
 from __future__ import annotations
 
@@ -42,6 +43,8 @@ __all__ = [
     "get_uv_of_triangle_forward",
     "get_uv_of_triangle_backward",
     "extract_bitfield_forward",
+    "sample_envmap_forward",
+    "blur_envmap_forward",
 ]
 
 
@@ -522,7 +525,47 @@ def extract_bitfield_forward(bitfield, N_bit, i_start, i_end):
     return out
 
 
-# ---- reference bindings outside this library (bindings.cpp:20-24, 31, etc.cu) ---------
+# ---- env-map preprocessing (bindings.cpp:20, 22, etc.cu) --------------------------------
+
+def _envmap_shape(envmap):
+    _check_float(envmap, "envmap")
+    if envmap.dim() != 3:
+        raise RuntimeError(f"envmap must have shape (C, H, W), got {tuple(envmap.shape)}")
+    return (int(x) for x in envmap.shape)
+
+
+def sample_envmap_forward(query_point, envmap):
+    """Bilinear lookup of envmap (C x H x W) along each direction of
+    query_point (3 x P, normalised here), etc.cu:1147-1178: returns C x P (the
+    reference allocates 3 x P whatever C is)."""
+    C, H, W = _envmap_shape(envmap)
+    _check_float(query_point, "query_point")
+    if query_point.dim() != 2 or query_point.shape[0] != 3:
+        raise RuntimeError(f"query_point must have shape (3, P), got {tuple(query_point.shape)}")
+    dev = envmap.device
+    _same_device(dev, [("query_point", query_point)])
+    P = int(query_point.shape[1])
+    with torch.cuda.device(dev):
+        out = torch.empty(C, P, dtype=torch.float32, device=dev)
+        _lib.call("des_sample_envmap_forward", P, C, H, W, _ptr(query_point), _ptr(envmap),
+                  _ptr(out), _stream(dev))
+    return out
+
+
+def blur_envmap_forward(envmap, tar_H, tar_W, N_samples, alpha):
+    """GGX blur of envmap (C x H x W, C = 3 or 4) onto a tar_H x tar_W map with
+    N_samples directions per texel, etc.cu:1103-1145: returns C x tar_H x tar_W."""
+    C, H, W = _envmap_shape(envmap)
+    dev = envmap.device
+    tar_H, tar_W, N_samples = int(tar_H), int(tar_W), int(N_samples)
+    with torch.cuda.device(dev):
+        out = torch.empty(C, max(tar_H, 0), max(tar_W, 0), dtype=torch.float32, device=dev)
+        _lib.call("des_blur_envmap_forward", C, H, W, tar_H, tar_W, N_samples, float(alpha),
+                  _ptr(envmap), _ptr(out), _stream(dev))
+    return out
+
+
+# ---- reference bindings outside this library (bindings.cpp:24, 31, etc.cu) -------------
 
 def _not_here(name):
     def fn(*args, **kwargs):
@@ -532,7 +575,5 @@ def _not_here(name):
     return fn
 
 
-blur_envmap_forward = _not_here("blur_envmap_forward")
-sample_envmap_forward = _not_here("sample_envmap_forward")
 generate_camera_ray_forward = _not_here("generate_camera_ray_forward")
 mt_format_conversion_forward = _not_here("mt_format_conversion_forward")

@@ -51,6 +51,8 @@ _SIGS = {
     "des_get_uv_of_triangle_forward": [_i64] + [_vp] * 3 + [_vp],
     "des_get_uv_of_triangle_backward": [_i64] + [_vp] * 4 + [_vp],
     "des_extract_bitfield_forward": [_vp, _i32, _i64, _i32, _i64, _i64, _vp, _vp],
+    "des_sample_envmap_forward": [_i64, _i32, _i32, _i32, _vp, _vp, _vp, _vp],
+    "des_blur_envmap_forward": [_i32] * 6 + [ctypes.c_float, _vp, _vp, _vp],
 }
 
 ERROR_NAMES = {

@@ -20,7 +20,8 @@ LIB_NAME = "libdiffenvshade.so"
 LIB_PATH = os.path.join(LIB_DIR, LIB_NAME)
 
 SOURCES = ["envshade_kernels.hip", "point_order.hip", "envshade_capi.cpp", "relight_kernels.hip",
-           "relight_capi.cpp", "mesh_kernels.hip", "mesh_capi.cpp"]
+           "relight_capi.cpp", "mesh_kernels.hip", "mesh_capi.cpp", "envmap_kernels.hip",
+           "envmap_capi.cpp"]
 HEADERS = ["envshade_internal.h", os.path.join("..", "..", "include", "diff_envshade.h")]
 
 ARCH = "gfx950"

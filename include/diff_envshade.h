@@ -2,8 +2,9 @@
  * diff_envshade.h -- C ABI of the MI355X (gfx950) brute-force environment-light
  * shader.  Drop-in boundary for the ten shader bindings of zjumsj/diff_envshade
  * (bindings.cpp:6-19, prototypes gsface_shader.h:3-118, host wrappers
- * bruteforce_shader.cu:2687-3453) and for the five mesh utilities that produce
- * its mesh inputs (bindings.cpp:27-30, 33; etc.cu), declared at the end.
+ * bruteforce_shader.cu:2687-3453), for the five mesh utilities that produce
+ * its mesh inputs (bindings.cpp:27-30, 33; etc.cu) and for the two env-map
+ * utilities beside it (bindings.cpp:20, 22; etc.cu), declared at the end.
  *
  * Conventions (all entry points):
  *   - every pointer is a device pointer to contiguous row-major data on the
@@ -372,6 +373,30 @@ DES_API int des_get_uv_of_triangle_backward(int64_t P, const float* triangles,
 DES_API int des_extract_bitfield_forward(const int32_t* bitfield, int32_t N_slot, int64_t N,
                                          int32_t N_bit, int64_t i_start, int64_t i_end,
                                          float* output, void* stream);
+
+/* ---- env-map preprocessing beside the shader ----
+ * FP64 directions, angles, bilinear weights and sums; row 0 is the top (+y),
+ * the seam at phi = 0 wraps (the reference's config.h). */
+
+/* sample_envmap_forward (etc.cu:1147-1178, kernel etc.cu:689-705): bilinear
+ * lookup of envmap (C x H x W) along each query direction (query_point 3 x P,
+ * normalised here).  output is C x P (the reference allocates 3 x P whatever
+ * C is, etc.cu:1164); a zero-length query samples NaN. */
+DES_API int des_sample_envmap_forward(int64_t P, int32_t C, int32_t H, int32_t W,
+                                      const float* query_point, const float* envmap, float* output,
+                                      void* stream);
+
+/* blur_envmap_forward (etc.cu:1103-1145, kernel etc.cu:607-687): each texel of
+ * a tar_H x tar_W map is the mean of N_samples bilinear lookups along
+ * GGX-distributed directions (roughness parameter alpha) around its centre
+ * direction; C must be 3 or 4 (DES_ERR_UNSUPPORTED_CHANNELS, "Channel number
+ * not supported!").  output is C x tar_H x tar_W.  The random numbers come
+ * from Philox4x32-10 (key 0, counter = sample, target texel), where the
+ * reference uses curand's XORWOW: the same estimator with other samples.
+ * The result is reproducible bit for bit.  N_samples must be >= 1. */
+DES_API int des_blur_envmap_forward(int32_t C, int32_t H, int32_t W, int32_t tar_H, int32_t tar_W,
+                                    int32_t N_samples, float alpha, const float* envmap,
+                                    float* output, void* stream);
 
 /* Timing hook for benchmarks (no counterpart in the reference): when set,
  * the calling thread's next *_backward* call records hipEvent_t `start` right

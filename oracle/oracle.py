@@ -28,7 +28,7 @@ def build(force: bool = False) -> str:
     """Compile the oracle with its Makefile (gcc + OpenMP)."""
     if force or not os.path.exists(_LIB_PATH) or any(
         os.path.getmtime(_LIB_PATH) < os.path.getmtime(os.path.join(_HERE, src))
-        for src in ("des_oracle.c", "mesh_oracle.c")
+        for src in ("des_oracle.c", "mesh_oracle.c", "envmap_oracle.c")
     ):
         subprocess.run(["make", "-B" if force else "-s", "-C", _HERE, "liboracle_des.so"],
                        check=True, stdout=subprocess.DEVNULL)
@@ -74,6 +74,16 @@ def lib():
                                                   dp]
         L.des_oracle_extract_bitfield.restype = None
         L.des_oracle_extract_bitfield.argtypes = [u32p, c_i64, c_int, c_i64, c_i64, dp]
+        fp = ctypes.POINTER(ctypes.c_float)
+        L.des_oracle_philox.restype = None
+        L.des_oracle_philox.argtypes = [u32p, u32p, u32p]
+        L.des_oracle_sample_envmap.restype = None
+        L.des_oracle_sample_envmap.argtypes = [c_i64, c_int, c_int, c_int, dp, fp, dp]
+        L.des_oracle_ggx_sample.restype = None
+        L.des_oracle_ggx_sample.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double, dp]
+        L.des_oracle_blur_envmap.restype = None
+        L.des_oracle_blur_envmap.argtypes = [c_int, c_int, c_int, c_int, c_int, c_int,
+                                             ctypes.c_double, fp, dp]
         _lib = L
     return _lib
 
@@ -254,4 +264,47 @@ def extract_bitfield(bitfield, N_bit, i_start, i_end):
     out = np.zeros((max(i_end - i_start, 0), N_bit))
     lib().des_oracle_extract_bitfield(bp, N, N_bit, i_start, i_end,
                                       out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    return out
+
+
+# ---- env-map utilities (envmap_oracle.c; etc.cu of the reference) ----------------------
+
+def philox(counter, key):
+    """Philox4x32-10 of a 4-word counter and a 2-word key -> 4 words."""
+    c = np.ascontiguousarray(counter, dtype=np.uint32)
+    k = np.ascontiguousarray(key, dtype=np.uint32)
+    out = np.zeros(4, dtype=np.uint32)
+    u32p = ctypes.POINTER(ctypes.c_uint32)
+    lib().des_oracle_philox(c.ctypes.data_as(u32p), k.ctypes.data_as(u32p), out.ctypes.data_as(u32p))
+    return out
+
+
+def sample_envmap(query_point, envmap):
+    """query_point 3 x P, envmap C x H x W -> C x P (FP64)."""
+    q, qp = _d(query_point)
+    e = np.ascontiguousarray(envmap, dtype=np.float32)
+    C, H, W = e.shape
+    P = q.shape[1]
+    out = np.zeros((C, P))
+    lib().des_oracle_sample_envmap(P, C, H, W, qp, e.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                   out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    return out
+
+
+def ggx_sample(r0, r1, alpha):
+    """GGX half-vector in the local frame (z up) from two uniforms."""
+    out = np.zeros(3)
+    lib().des_oracle_ggx_sample(float(r0), float(r1), float(alpha),
+                                out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    return out
+
+
+def blur_envmap(envmap, tar_H, tar_W, N_samples, alpha):
+    """envmap C x H x W -> C x tar_H x tar_W (FP64), Philox4x32-10 samples."""
+    e = np.ascontiguousarray(envmap, dtype=np.float32)
+    C, H, W = e.shape
+    out = np.zeros((C, tar_H, tar_W))
+    lib().des_oracle_blur_envmap(C, H, W, tar_H, tar_W, N_samples, float(np.float32(alpha)),
+                                 e.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                 out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
     return out

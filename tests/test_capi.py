@@ -254,3 +254,26 @@ def test_python_extract_bitfield_reports_invalid_ranges():
     # the checks run before any device work, so a CPU tensor is rejected first
     with pytest.raises(RuntimeError, match="CUDA"):
         de.extract_bitfield_forward(torch.zeros(2, 4, dtype=torch.int32), 8, 0, 4)
+
+
+# ---- env-map utilities ---------------------------------------------------------------
+
+@pytest.mark.parametrize("args,code", [
+    ((5, 8, 16, 4, 8, 10, 0.1), -2),   # C must be 3 or 4
+    ((3, 0, 16, 4, 8, 10, 0.1), -1),
+    ((3, 8, 16, -1, 8, 10, 0.1), -1),
+    ((3, 8, 16, 4, 8, 0, 0.1), -1),    # at least one sample
+    ((3, 8, 16, 4, 8, 10, float("nan")), -1),
+    ((3, 8, 16, 0, 8, 10, 0.1), 0),    # empty target: nothing to do
+])
+def test_blur_validation(args, code):
+    rc = _lib.load().des_blur_envmap_forward(*args, DUMMY, DUMMY, None)
+    assert rc == code, _lib.load().des_last_error()
+
+
+def test_sample_validation():
+    L = _lib.load()
+    assert L.des_sample_envmap_forward(-1, 3, 8, 16, DUMMY, DUMMY, DUMMY, None) == -1
+    assert L.des_sample_envmap_forward(10, 0, 8, 16, DUMMY, DUMMY, DUMMY, None) == -1
+    assert L.des_sample_envmap_forward(10, 3, 8, 16, None, DUMMY, DUMMY, None) == -1
+    assert L.des_sample_envmap_forward(0, 3, 8, 16, None, None, None, None) == 0

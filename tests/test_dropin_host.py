@@ -63,8 +63,7 @@ def test_cpu_tensors_are_rejected_like_torch_check():
                                                          None, None, True)
 
 
-@pytest.mark.parametrize("name", ["blur_envmap_forward", "sample_envmap_forward",
-                                  "generate_camera_ray_forward", "mt_format_conversion_forward"])
+@pytest.mark.parametrize("name", ["generate_camera_ray_forward", "mt_format_conversion_forward"])
 def test_non_shader_bindings_are_outside_the_library(name):
     with pytest.raises(NotImplementedError):
         getattr(C, name)()
@@ -83,6 +82,10 @@ def test_mesh_bindings_check_their_tensors():
         C.get_uv_of_triangle_backward(torch.zeros(4, 3, 3), torch.zeros(4, 3), torch.zeros(4, 3))
     with pytest.raises(RuntimeError, match="tensor"):
         C.extract_bitfield_forward(None, 8, 0, 1)
+    with pytest.raises(RuntimeError, match="envmap must be a CUDA tensor"):
+        C.sample_envmap_forward(torch.zeros(3, 4), torch.zeros(3, 2, 4))
+    with pytest.raises(RuntimeError, match="envmap must be a CUDA tensor"):
+        C.blur_envmap_forward(torch.zeros(3, 2, 4), 2, 4, 8, 0.1)
 
 
 def test_autograd_classes_exist():
