@@ -109,24 +109,29 @@ __global__ __launch_bounds__(256) void sample_envmap_kernel(int64_t P, int C, in
   for (int c = 0; c < C; ++c) out[c * P + p] = (float)blend(t, env + c * plane);
 }
 
-// BlurEnvmap_Forward (etc.cu:607-687): one wave per target texel.  Its lanes
-// take samples lane, lane + 64, ...; each adds its GGX-distributed lookups in
-// FP64, and the wave sums the lanes in a fixed tree order, so the result is
-// reproducible bit for bit.
-template <int C>
+// BlurEnvmap_Forward (etc.cu:607-687): WPT waves per target texel (1 or all 4
+// of the block).  The texel's lanes take samples l, l + 64 WPT, ...; each adds
+// its GGX-distributed lookups in FP64, each wave sums its lanes in a fixed tree
+// order and the block sums its waves in order, so the result is reproducible
+// bit for bit.
+template <int C, int WPT>
 __global__ __launch_bounds__(256) void blur_envmap_kernel(int H, int W, int tar_H, int tar_W,
                                                           int N, float alpha,
                                                           const float* __restrict__ env,
                                                           float* __restrict__ out) {
-  const int lane = threadIdx.x & 63;
+  __shared__ double part[4][C];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int sub = wv % WPT;  // wave within the texel
   const int64_t T = (int64_t)tar_H * tar_W;
-  const int64_t idx = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (idx >= T) return;  // whole waves
+  const int64_t idx = (int64_t)blockIdx.x * (4 / WPT) + wv / WPT;
+  if (idx >= T) return;  // whole waves; with WPT = 4 the grid holds exactly T blocks
   const int ih = (int)(idx / tar_W), iw = (int)(idx % tar_W);
   // the texel's centre direction and a frame around it (etc.cu:616-656)
   const double theta = (ih + 0.5) / tar_H * kPi;
   const double phi = (iw + 0.5) / tar_W * 2.0 * kPi;
-  const double st = sin(theta), ct = cos(theta), sp = sin(phi), cp = cos(phi);
+  double st, ct, sp, cp;
+  sincos(theta, &st, &ct);
+  sincos(phi, &sp, &cp);
   double n[3] = {st * cp, ct, st * sp};
   double L = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
   n[0] /= L;
@@ -157,7 +162,7 @@ __global__ __launch_bounds__(256) void blur_envmap_kernel(int H, int W, int tar_
   double acc[C];
 #pragma unroll
   for (int c = 0; c < C; ++c) acc[c] = 0.0;
-  for (int i = lane; i < N; i += 64) {
+  for (int i = sub * 64 + lane; i < N; i += 64 * WPT) {
     uint32_t r[4] = {(uint32_t)i, (uint32_t)idx, 0u, 0u};
     philox4x32_10(r, 0u, 0u);
     // TrowbridgeReitzDistributionSample (etc.cu:588-605)
@@ -168,7 +173,9 @@ __global__ __launch_bounds__(256) void blur_envmap_kernel(int H, int W, int tar_
     double tmp = 1.0 - cos_t * cos_t;
     if (tmp < 0.0) tmp = 0.0;
     const double sin_t = sqrt(tmp);
-    const double l0 = sin_t * cos(ps), l1 = sin_t * sin(ps), l2 = cos_t;
+    double sps, cps;
+    sincos(ps, &sps, &cps);
+    const double l0 = sin_t * cps, l1 = sin_t * sps, l2 = cos_t;
     double d[3];
 #pragma unroll
     for (int j = 0; j < 3; ++j) d[j] = vx[j] * l0 + vy[j] * l1 + n[j] * l2;
@@ -181,6 +188,20 @@ __global__ __launch_bounds__(256) void blur_envmap_kernel(int H, int W, int tar_
   for (int c = 0; c < C; ++c) {
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) acc[c] += __shfl_xor(acc[c], d, 64);
+  }
+  if (WPT > 1) {
+    if (lane == 0) {
+#pragma unroll
+      for (int c = 0; c < C; ++c) part[wv][c] = acc[c];
+    }
+    __syncthreads();
+    if (sub != 0) return;
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      double s = part[wv][c];
+      for (int w = 1; w < WPT; ++w) s += part[wv + w][c];
+      acc[c] = s;
+    }
   }
   if (lane == 0) {
 #pragma unroll
@@ -198,17 +219,30 @@ hipError_t launch_sample_envmap(int64_t P, int C, int H, int W, const float* q, 
   return hipGetLastError();
 }
 
+// Waves per target texel: all four of a block when one wave per texel would
+// leave most SIMDs without a wave and each texel has samples enough for four
+// waves (at 2048 texels one wave per texel is as fast).
 hipError_t launch_blur_envmap(int C, int H, int W, int tar_H, int tar_W, int N, float alpha,
                               const float* env, float* out, hipStream_t stream) {
   const int64_t T = (int64_t)tar_H * tar_W;
   if (T <= 0) return hipSuccess;
-  const dim3 grid((unsigned)((T + 3) / 4));
-  if (C == 3)
-    hipLaunchKernelGGL(blur_envmap_kernel<3>, grid, dim3(256), 0, stream, H, W, tar_H, tar_W, N,
-                       alpha, env, out);
-  else
-    hipLaunchKernelGGL(blur_envmap_kernel<4>, grid, dim3(256), 0, stream, H, W, tar_H, tar_W, N,
-                       alpha, env, out);
+  const bool wide = T < 1024 && N >= 256;
+  const dim3 grid((unsigned)(wide ? T : (T + 3) / 4));
+  if (C == 3) {
+    if (wide)
+      hipLaunchKernelGGL((blur_envmap_kernel<3, 4>), grid, dim3(256), 0, stream, H, W, tar_H,
+                         tar_W, N, alpha, env, out);
+    else
+      hipLaunchKernelGGL((blur_envmap_kernel<3, 1>), grid, dim3(256), 0, stream, H, W, tar_H,
+                         tar_W, N, alpha, env, out);
+  } else {
+    if (wide)
+      hipLaunchKernelGGL((blur_envmap_kernel<4, 4>), grid, dim3(256), 0, stream, H, W, tar_H,
+                         tar_W, N, alpha, env, out);
+    else
+      hipLaunchKernelGGL((blur_envmap_kernel<4, 1>), grid, dim3(256), 0, stream, H, W, tar_H,
+                         tar_W, N, alpha, env, out);
+  }
   return hipGetLastError();
 }
 

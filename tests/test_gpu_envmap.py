@@ -57,6 +57,8 @@ def test_sample_edge_cases():
     (3, 8, 16, 3, 5, 65, 1.0),     # one sample more than lanes
     (3, 32, 64, 32, 64, 1, 0.5),
     (4, 16, 32, 16, 32, 16, 0.0),  # zero roughness at the source size: the map itself
+    (4, 16, 32, 6, 10, 300, 0.6),  # four waves per target texel
+    (3, 8, 16, 2, 3, 256, 0.2),
 ])
 def test_blur_matches_oracle(C, H, W, tH, tW, N, alpha):
     rng = np.random.default_rng(N)
