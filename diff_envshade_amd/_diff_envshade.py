@@ -23,6 +23,8 @@ NotImplementedError.
 """
 from __future__ import annotations
 
+import weakref
+
 import torch
 
 from . import _lib
@@ -114,11 +116,29 @@ def _ptr(x):
     return x.data_ptr() if (x is not None and x.numel() > 0) else None
 
 
-def _workspace(P: int, H: int, W: int, dev: torch.device):
+# The last forward's point order per device, for the backward that usually
+# follows with the same normals: (weak reference to normal, its data pointer,
+# its version counter, P, stream, the forward workspace holding the order).
+_order_cache: dict = {}
+
+
+def _workspace(P: int, H: int, W: int, dev: torch.device, normal: torch.Tensor):
+    """Backward workspace.  When the device's last forward sorted these very
+    normals -- the same tensor (alive, same data pointer, unchanged version
+    counter) and P, on the current stream -- the library is told to take its
+    point order instead of sorting again."""
     n = _lib.workspace_size(P, H, W)
     if n == 0:
         return None, 0
-    return torch.empty(n, dtype=torch.uint8, device=dev), n
+    ws = torch.empty(n, dtype=torch.uint8, device=dev)
+    e = _order_cache.get(dev.index)
+    if e is not None:
+        ref, ptr, ver, p, stream, ows = e
+        src = ref()
+        if (src is not None and p == P and ptr == normal.data_ptr() == src.data_ptr()
+                and ver == src._version == normal._version and stream == _stream(dev)):
+            _lib.reuse_point_order_once(ows.data_ptr())
+    return ws, n
 
 
 def _fwd_workspace(P: int, H: int, W: int, dev: torch.device):
@@ -128,6 +148,14 @@ def _fwd_workspace(P: int, H: int, W: int, dev: torch.device):
     if n == 0:
         return None, 0
     return torch.empty(n, dtype=torch.uint8, device=dev), n
+
+
+def _remember_order(ws, P: int, dev: torch.device, normal: torch.Tensor) -> None:
+    """After a forward call that sorted into `ws` has been enqueued: keep its
+    order for the backward (the call raised otherwise, so the order is real)."""
+    if ws is not None:
+        _order_cache[dev.index] = (weakref.ref(normal), normal.data_ptr(), normal._version, P,
+                                   _stream(dev), ws)
 
 
 def _material_checks(P, C, normal, view_dir, albedo, specular_albedo, roughness, envmap,
@@ -188,6 +216,7 @@ def bruteforce_diffuse_shader_forward(normal, albedo, envmap, light2obj_rotmat):
         _lib.call("des_bruteforce_diffuse_shader_forward", P, C, H, W, _ptr(normal), _ptr(albedo),
                   _ptr(envmap), _ptr(light2obj_rotmat), _ptr(shading), _ptr(ws), nws,
                   _stream(dev))
+        _remember_order(ws, P, dev, normal)
     return shading
 
 
@@ -205,7 +234,7 @@ def bruteforce_diffuse_shader_backward(grad_shading, normal, albedo, envmap, lig
     grad_albedo = torch.empty((P, C), dtype=normal.dtype, device=dev)
     grad_envmap = torch.empty((C, H, W), dtype=normal.dtype, device=dev)
     with torch.cuda.device(dev):
-        ws, nws = _workspace(P, H, W, dev)
+        ws, nws = _workspace(P, H, W, dev, normal)
         _lib.call("des_bruteforce_diffuse_shader_backward", P, C, H, W, _ptr(grad_shading),
                   _ptr(normal), _ptr(albedo), _ptr(envmap), _ptr(light2obj_rotmat),
                   _ptr(grad_normal), _ptr(grad_albedo), _ptr(grad_envmap), _ptr(ws), nws,
@@ -231,6 +260,7 @@ def _spec_forward(entry, normal, view_dir, albedo, specular_albedo, roughness, e
                   _ptr(specular_albedo), _ptr(roughness), _ptr(envmap), _ptr(light2obj_rotmat),
                   int(bool(enable_diffuse)), int(bool(enable_specular)), _ptr(shading),
                   _ptr(ws), nws, _stream(dev))
+        _remember_order(ws, P, dev, normal)
     return shading
 
 
@@ -246,7 +276,7 @@ def _spec_backward(entry, grad_shading, normal, view_dir, albedo, specular_albed
     _grad_checks(P, C, dev, (("grad_shading", grad_shading),))
     outs = _alloc_grads(P, C, H, W, normal)
     with torch.cuda.device(dev):
-        ws, nws = _workspace(P, H, W, dev)
+        ws, nws = _workspace(P, H, W, dev, normal)
         _lib.call(entry, P, C, H, W, _ptr(grad_shading), _ptr(normal), _ptr(view_dir),
                   _ptr(albedo), _ptr(specular_albedo), _ptr(roughness), _ptr(envmap),
                   _ptr(light2obj_rotmat), int(bool(enable_diffuse)), int(bool(enable_specular)),
@@ -314,6 +344,7 @@ def bruteforce_specular_shader_forward2(normal, view_dir, albedo, specular_albed
                   _ptr(view_dir), _ptr(albedo), _ptr(specular_albedo), _ptr(roughness),
                   _ptr(envmap), _ptr(light2obj_rotmat), int(bool(enable_specular)),
                   _ptr(shading), _ptr(diffuse_shading), _ptr(ws), nws, _stream(dev))
+        _remember_order(ws, P, dev, normal)
     return [shading, diffuse_shading]
 
 
@@ -331,7 +362,7 @@ def bruteforce_specular_shader_backward2(grad_shading, grad_diffuse_shading, nor
                              ("grad_diffuse_shading", grad_diffuse_shading)))
     outs = _alloc_grads(P, C, H, W, normal)
     with torch.cuda.device(dev):
-        ws, nws = _workspace(P, H, W, dev)
+        ws, nws = _workspace(P, H, W, dev, normal)
         _lib.call("des_bruteforce_specular_shader_backward2", P, C, H, W, _ptr(grad_shading),
                   _ptr(grad_diffuse_shading), _ptr(normal), _ptr(view_dir), _ptr(albedo),
                   _ptr(specular_albedo), _ptr(roughness), _ptr(envmap), _ptr(light2obj_rotmat),
@@ -366,6 +397,7 @@ def bruteforce_specularvisibility_shader_forward2(normal, view_dir, albedo, spec
                   _ptr(nearest_triangle_id), _ptr(barycentric_coord), _ptr(visibility),
                   int(bool(enable_specular)), _ptr(shading), _ptr(diffuse_shading), _ptr(ws),
                   nws, _stream(dev))
+        _remember_order(ws, P, dev, normal)
     return [shading, diffuse_shading]
 
 
@@ -387,7 +419,7 @@ def bruteforce_specularvisibility_shader_backward2(grad_shading, grad_diffuse_sh
                              visibility)
     outs = _alloc_grads(P, C, H, W, normal)
     with torch.cuda.device(dev):
-        ws, nws = _workspace(P, H, W, dev)
+        ws, nws = _workspace(P, H, W, dev, normal)
         _lib.call("des_bruteforce_specularvisibility_shader_backward2", P, C, H, W, NF, NV, NS,
                   _ptr(grad_shading), _ptr(grad_diffuse_shading), _ptr(normal), _ptr(view_dir),
                   _ptr(albedo), _ptr(specular_albedo), _ptr(roughness), _ptr(envmap),

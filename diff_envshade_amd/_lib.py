@@ -107,6 +107,8 @@ def load() -> ctypes.CDLL:
         L.des_set_tiling_override.restype = _int
         L.des_set_backward_timing_events.argtypes = [_vp, _vp]
         L.des_set_backward_timing_events.restype = _int
+        L.des_reuse_point_order_once.argtypes = [_vp]
+        L.des_reuse_point_order_once.restype = _int
         _lib = L
     return _lib
 
@@ -127,6 +129,12 @@ def workspace_size(P: int, H: int, W: int) -> int:
 
 def forward_workspace_size(P: int, H: int, W: int) -> int:
     return int(load().des_forward_workspace_size(P, H, W))
+
+
+def reuse_point_order_once(order_workspace_ptr) -> None:
+    """The calling thread's next shading call takes its point order from this
+    forward workspace (see des_reuse_point_order_once)."""
+    check(load().des_reuse_point_order_once(order_workspace_ptr))
 
 
 def set_point_order(mode: int) -> None:

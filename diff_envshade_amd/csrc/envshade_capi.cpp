@@ -19,8 +19,12 @@
 namespace des_capi {
 
 thread_local std::string g_last_error;
+// Point order handed to the calling thread's next call (des_reuse_point_order_once);
+// every call returning through fail() or ok() drops it, the shading calls take it.
+thread_local const void* g_reuse_order = nullptr;
 
 int fail(int code, const char* fmt, ...) {
+  g_reuse_order = nullptr;
   char buf[512];
   va_list ap;
   va_start(ap, fmt);
@@ -31,8 +35,15 @@ int fail(int code, const char* fmt, ...) {
 }
 
 int ok() {
+  g_reuse_order = nullptr;
   g_last_error.clear();
   return DES_OK;
+}
+
+const void* take_reuse_order() {
+  const void* r = g_reuse_order;
+  g_reuse_order = nullptr;
+  return r;
 }
 
 }  // namespace des_capi
@@ -53,12 +64,13 @@ constexpr int kTileP = 64;
 constexpr int64_t kMaxTexels = int64_t(1) << 24;
 constexpr int kMinWaves = 1024;  // one wave per SIMD on 256 CUs x 4 SIMDs
 // Automatic point order from this many point-texel pairs up: below it the
-// three sort launches (~15 us) cost about what the skipped texels save.  The
-// forward, at a third of the backward's work per pair, needs larger problems
-// (measured on MI355X: 70k x 512 pairs shade faster unsorted in the forward,
-// 150k x 2048 faster sorted in both directions).
+// three sort launches (~15 us) cost about what the skipped texels save.  In
+// both directions the same threshold, so that a forward's order can serve the
+// backward that follows it (des_reuse_point_order_once); alone, the forward
+// about breaks even there (measured on MI355X at 70k x 512 pairs: 81.0 us
+// unsorted, 79.5 us sorted including the sort).
 constexpr int64_t kOrderMinPairsBackward = int64_t(1) << 24;
-constexpr int64_t kOrderMinPairsForward = int64_t(1) << 26;
+constexpr int64_t kOrderMinPairsForward = int64_t(1) << 24;
 
 int64_t n_tiles(int64_t P) { return (P + kTileP - 1) / kTileP; }
 
@@ -175,8 +187,12 @@ int launch_error(hipError_t e, const char* what) {
 
 int run_forward(int variant, bool diff, bool spec, des::Problem& pr, void* ws, size_t ws_bytes,
                 hipStream_t stream) {
+  const void* reuse = des_capi::take_reuse_order();
   if (pr.P == 0) return ok();
-  if (ws != nullptr && use_order(pr.P, pr.T, false)) {
+  if (reuse != nullptr && use_order(pr.P, pr.T, false)) {
+    pr.order = reinterpret_cast<const int32_t*>(static_cast<const char*>(reuse) +
+                                                des::order_layout(pr.P).order_off);
+  } else if (ws != nullptr && use_order(pr.P, pr.T, false)) {
     const des::OrderLayout ol = des::order_layout(pr.P);
     if (ws_bytes < ol.total)
       return fail(DES_ERR_WORKSPACE_TOO_SMALL, "workspace of %zu bytes is smaller than the %zu required",
@@ -195,6 +211,7 @@ int run_forward(int variant, bool diff, bool spec, des::Problem& pr, void* ws, s
 
 int run_backward(int variant, bool diff, bool spec, des::Problem& pr, void* ws, size_t ws_bytes,
                  hipStream_t stream) {
+  const void* reuse = des_capi::take_reuse_order();
   REQUIRE(pr.gE);
   if (pr.P == 0) {
     hipError_t e = des::launch_fill_zero(pr.gE, int64_t(3) * pr.T, stream);
@@ -209,7 +226,10 @@ int run_backward(int variant, bool diff, bool spec, des::Problem& pr, void* ws, 
     return fail(DES_ERR_INVALID_ARGUMENT, "workspace must be 256-byte aligned");
   pr.part = reinterpret_cast<float*>(static_cast<char*>(ws) + lay.part_off);
   pr.part2 = reinterpret_cast<double*>(static_cast<char*>(ws) + lay.part2_off);
-  if (lay.ordered) {
+  if (lay.ordered && reuse != nullptr) {
+    pr.order = reinterpret_cast<const int32_t*>(static_cast<const char*>(reuse) +
+                                                des::order_layout(pr.P).order_off);
+  } else if (lay.ordered) {
     char* ows = static_cast<char*>(ws) + lay.order_off;
     hipError_t e = des::launch_point_order(pr.normal, pr.P, ows, stream);
     if (e != hipSuccess) return launch_error(e, "point order failed");
@@ -369,6 +389,14 @@ DES_API size_t des_backward_workspace_size(int64_t P, int32_t H, int32_t W) {
 DES_API size_t des_forward_workspace_size(int64_t P, int32_t H, int32_t W) {
   if (P <= 0 || H <= 0 || W <= 0 || !use_order(P, H * W, false)) return 0;
   return des::order_layout(P).total;
+}
+
+DES_API int des_reuse_point_order_once(const void* order_workspace) {
+  if ((reinterpret_cast<uintptr_t>(order_workspace) & 255) != 0)
+    return fail(DES_ERR_INVALID_ARGUMENT, "order workspace must be 256-byte aligned");
+  const int rc = ok();
+  des_capi::g_reuse_order = order_workspace;
+  return rc;
 }
 
 DES_API int des_set_point_order(int32_t mode) {

@@ -81,10 +81,22 @@ DES_API size_t des_backward_workspace_size(int64_t P, int32_t H, int32_t W);
 DES_API size_t des_forward_workspace_size(int64_t P, int32_t H, int32_t W);
 
 /* Point-order policy (no counterpart in the reference), process-wide:
- * 0 = automatic (the backward reorders from 2^24 point-texel pairs P x H x W,
- * the forward from 2^26), 1 = always, -1 = never.
+ * 0 = automatic (both directions reorder from 2^24 point-texel pairs
+ * P x H x W), 1 = always, -1 = never.
  * The workspace sizes above follow the policy in force when they are asked. */
 DES_API int des_set_point_order(int32_t mode);
+
+/* Point order shared between calls (no counterpart in the reference).  The
+ * calling thread's next call into the library, if it is a shading entry point
+ * that would reorder its points, takes the order from `order_workspace`
+ * instead of sorting: the workspace of an earlier *_forward* call that sorted
+ * (des_forward_workspace_size bytes, 256-byte aligned) with the same P,
+ * ordered before this call on the stream.  With the same normals the outputs
+ * are bitwise those of a call that sorts by itself; every order is a
+ * permutation, so with other normals they stay correct and move at rounding
+ * level only.  The next call consumes the hook whatever it is; NULL clears
+ * it. */
+DES_API int des_reuse_point_order_once(const void* order_workspace);
 
 /* bindings.cpp:6 -> bruteforce_diffuse_shader_forward (bruteforce_shader.cu:2687-2731).
  * Lambert only: shading = sum_t [l.n>0] E a (l.n/pi) dOmega. */

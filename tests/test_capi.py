@@ -111,10 +111,12 @@ def test_forward_workspace_follows_point_order_policy():
         order = _lib.forward_workspace_size(70000, 16, 32)
         assert _lib.workspace_size(70000, 16, 32) == never + order
         _lib.set_point_order(0)
-        # automatic: the backward reorders from 2^24 pairs, the forward from 2^26
-        assert _lib.forward_workspace_size(70000, 16, 32) == 0  # 35.8 M pairs
+        # automatic: both directions reorder from 2^24 pairs
+        assert _lib.forward_workspace_size(70000, 16, 32) == order  # 35.8 M pairs
         assert _lib.workspace_size(70000, 16, 32) == never + order
-        assert _lib.forward_workspace_size(40000, 32, 64) > 0  # 81.9 M pairs
+        assert _lib.forward_workspace_size(32768, 16, 32) > 0  # 2^24 pairs
+        assert _lib.forward_workspace_size(32767, 16, 32) == 0
+        assert _lib.workspace_size(32767, 16, 32) < _lib.workspace_size(32768, 16, 32)
         assert _lib.forward_workspace_size(100, 4, 8) == 0
         with pytest.raises(RuntimeError):
             _lib.set_point_order(2)
@@ -277,3 +279,13 @@ def test_sample_validation():
     assert L.des_sample_envmap_forward(10, 0, 8, 16, DUMMY, DUMMY, DUMMY, None) == -1
     assert L.des_sample_envmap_forward(10, 3, 8, 16, None, DUMMY, DUMMY, None) == -1
     assert L.des_sample_envmap_forward(0, 3, 8, 16, None, None, None, None) == 0
+
+
+def test_reuse_point_order_hook_validation():
+    L = _lib.load()
+    assert L.des_reuse_point_order_once(ctypes.c_void_p(DUMMY + 4)) == -1
+    assert "256-byte aligned" in L.des_last_error().decode()
+    assert L.des_reuse_point_order_once(ctypes.c_void_p(DUMMY)) == 0
+    # any call consumes the hook: a failing one here, before any launch
+    assert _vis_fwd(C=4) == -2
+    assert L.des_reuse_point_order_once(None) == 0

@@ -74,3 +74,87 @@ def test_order_block_boundaries(n):
     (f0, b0), (f1, b1) = _run(-1, "vis", d), _run(1, "vis", d)
     for a, b in zip(f0 + b0, f1 + b1):
         assert relerr(b, a) <= TOL
+
+
+def _vis_args(t):
+    mat = [t["normal"], t["view_dir"], t["albedo"], t["specular_albedo"], t["roughness"],
+           t["envmap"], t["light2obj_rotmat"]]
+    vis = [t["face_vertex_list"], t["nearest_triangle_id"], t["barycentric_coord"],
+           t["visibility"]]
+    return mat, vis
+
+
+def _device(d):
+    import torch
+    return {k: torch.from_numpy(np.ascontiguousarray(v)).to("cuda:0") for k, v in d.items()}
+
+
+@pytest.fixture
+def reuse_spy(monkeypatch):
+    """Counts the order hand-overs the drop-in module makes."""
+    from diff_envshade_amd import _diff_envshade as de
+    from diff_envshade_amd import _lib
+    calls = []
+    real = _lib.reuse_point_order_once
+
+    def spy(ptr):
+        calls.append(ptr)
+        real(ptr)
+    monkeypatch.setattr(_lib, "reuse_point_order_once", spy)
+    de._order_cache.clear()
+    yield calls
+    de._order_cache.clear()
+
+
+def test_forward_order_serves_the_backward(reuse_spy):
+    # 40,000 points x 512 texels (2^24 pairs and more): both directions sort;
+    # a backward right after the forward takes the forward's order, and its
+    # outputs are bitwise those of a backward that sorts by itself
+    import torch
+    from diff_envshade_amd import _diff_envshade as de
+    t = _device(problem(seed=31, n=40000, H=16, W=32, vis="identity"))
+    mat, vis = _vis_args(t)
+    g = (t["grad_shading"], t["grad_diffuse_shading"])
+    alone = de.bruteforce_specularvisibility_shader_backward2(*g, *mat, *vis, True)
+    assert reuse_spy == []
+    de.bruteforce_specularvisibility_shader_forward2(*mat, *vis, True)
+    shared = de.bruteforce_specularvisibility_shader_backward2(*g, *mat, *vis, True)
+    assert len(reuse_spy) == 1
+    for a, b in zip(alone, shared):
+        assert torch.equal(a, b)
+    # an in-place change of the normals (version counter) stops the hand-over
+    t["normal"].mul_(1.0)
+    de.bruteforce_specularvisibility_shader_backward2(*g, *mat, *vis, True)
+    assert len(reuse_spy) == 1
+    # another variant's backward after the vis forward shares it too
+    de.bruteforce_specularvisibility_shader_forward2(*mat, *vis, True)
+    de.bruteforce_specular_shader_backward2(*g, *mat, True)
+    assert len(reuse_spy) == 2
+
+
+def test_an_order_of_other_normals_is_still_correct(reuse_spy):
+    # the hand-over of an order sorted from different normals of the same P is
+    # a permutation all the same: the outputs move by rounding only (the tiles
+    # differ, so each point's sums are grouped differently)
+    import torch
+    from diff_envshade_amd import _diff_envshade as de
+    d = problem(seed=32, n=40000, H=16, W=32, vis="identity")
+    t = _device(d)
+    mat, vis = _vis_args(t)
+    g = (t["grad_shading"], t["grad_diffuse_shading"])
+    fresh = de.bruteforce_specularvisibility_shader_backward2(*g, *mat, *vis, True)
+    other = torch.roll(t["normal"], 977, dims=0).contiguous()
+    de.bruteforce_specularvisibility_shader_forward2(other, *mat[1:], *vis, True)
+    ows = de._order_cache[0][5]
+    from diff_envshade_amd import _lib
+    _lib.reuse_point_order_once(ows.data_ptr())
+    P, H, W = 40000, 16, 32
+    outs = [torch.empty_like(x) for x in fresh]
+    ws = torch.empty(_lib.workspace_size(P, H, W), dtype=torch.uint8, device="cuda:0")
+    _lib.reuse_point_order_once(ows.data_ptr())
+    _lib.call("des_bruteforce_specularvisibility_shader_backward2", P, 3, H, W,
+              t["face_vertex_list"].shape[1], t["visibility"].shape[1], t["visibility"].shape[0],
+              *(x.data_ptr() for x in (*g, *mat, *vis)), 1, *(o.data_ptr() for o in outs),
+              ws.data_ptr(), ws.numel(), torch.cuda.current_stream().cuda_stream)
+    for a, b in zip(fresh, outs):
+        assert relerr(b.cpu().numpy(), a.cpu().numpy()) <= TOL
