@@ -246,14 +246,13 @@ def main() -> int:
     }
 
     if not (args.no_accuracy and args.no_cpu_baseline):
-        sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
-        from oracle import oracle as O
+        O = _oracle()
         sh, ds, g = out
         torch.cuda.synchronize()
         print("[bench] FP64 oracle on the host ...", file=sys.stderr, flush=True)
         runs = []
         ref = None
-        n_runs = 1 if args.no_cpu_baseline else 4
+        n_runs = 1 if args.no_cpu_baseline else 6
         for i in range(n_runs):
             a = time.perf_counter()
             rsh, rds = O.forward("vis", d)
@@ -279,7 +278,8 @@ def main() -> int:
             line["cpu_baseline"] = {
                 "value": P / med,
                 "unit": UNIT,
-                "cores": O.max_threads(),
+                "cores": O.max_threads(), "cpu_model": _cpu_model(),
+                "omp_proc_bind": os.environ.get("OMP_PROC_BIND"),
                 "kind": "port",
                 "sample": f"the full workload ({P} points x {T} texels, forward+backward, FP64 "
                           f"oracle, OpenMP), median of {len(timed)} runs after 1 warm-up",
@@ -292,9 +292,21 @@ def main() -> int:
 # ---- other workloads (not the headline) ---------------------------------------------
 
 def _oracle():
+    os.environ.setdefault("OMP_PROC_BIND", "close")  # read when the OpenMP runtime starts
     sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
     from oracle import oracle as O
     return O
+
+
+def _cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def _time_ms(fn, reps=10, warmup=2):
@@ -441,6 +453,8 @@ def bench_nearest_mesh(args):
             timed = sorted(runs[1:])
             med = timed[len(timed) // 2]
             line["cpu_baseline"] = {"value": n / med, "unit": "queries/s", "cores": O.max_threads(),
+                                    "cpu_model": _cpu_model(),
+                                    "omp_proc_bind": os.environ.get("OMP_PROC_BIND"),
                                     "kind": "port",
                                     "sample": f"the full workload ({n} queries, forward+backward, "
                                               f"FP64 oracle, OpenMP), median of {len(timed)} "
@@ -506,7 +520,7 @@ def bench_bitfield(args):
         if not args.no_cpu_baseline:
             timed = sorted(runs[1:])
             med = timed[len(timed) // 2]
-            line["cpu_baseline"] = {"value": n / med, "unit": "points/s", "cores": 1,
+            line["cpu_baseline"] = {"value": n / med, "unit": "points/s", "cores": 1, "cpu_model": _cpu_model(),
                                     "kind": "port",
                                     "sample": f"the full workload ({n} points x {N_bit} bits, "
                                               "scalar C oracle), median of "
@@ -674,7 +688,9 @@ def bench_envmap_sample(args):
             timed = sorted(runs[1:])
             med = timed[len(timed) // 2]
             line["cpu_baseline"] = {"value": P / med, "unit": "lookups/s",
-                                    "cores": O.max_threads(), "kind": "port",
+                                    "cores": O.max_threads(), "cpu_model": _cpu_model(),
+                                    "omp_proc_bind": os.environ.get("OMP_PROC_BIND"),
+                                    "kind": "port",
                                     "sample": f"the full workload ({P} lookups, FP64 oracle, "
                                               f"OpenMP), median of {len(timed)} runs after 1 "
                                               "warm-up", "seconds_per_step": med}
@@ -737,7 +753,9 @@ def bench_envmap_blur(args):
             timed = sorted(runs[1:])
             med = timed[len(timed) // 2]
             line["cpu_baseline"] = {"value": samples / med, "unit": "samples/s",
-                                    "cores": O.max_threads(), "kind": "port",
+                                    "cores": O.max_threads(), "cpu_model": _cpu_model(),
+                                    "omp_proc_bind": os.environ.get("OMP_PROC_BIND"),
+                                    "kind": "port",
                                     "sample": f"the full workload ({samples} samples, FP64 "
                                               f"oracle, OpenMP), median of {len(timed)} runs "
                                               "after 1 warm-up", "seconds_per_step": med}
