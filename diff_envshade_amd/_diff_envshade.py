@@ -123,22 +123,24 @@ _order_cache: dict = {}
 
 
 def _workspace(P: int, H: int, W: int, dev: torch.device, normal: torch.Tensor):
-    """Backward workspace.  When the device's last forward sorted these very
-    normals -- the same tensor (alive, same data pointer, unchanged version
-    counter) and P, on the current stream -- the library is told to take its
-    point order instead of sorting again."""
+    """Backward workspace, and the point order to hand over: when the device's
+    last forward sorted these very normals -- the same tensor (alive, same
+    data pointer, unchanged version counter) and P, on the current stream --
+    (its workspace pointer, P), which the call passes to the library so that it
+    takes that order instead of sorting again; else None."""
     n = _lib.workspace_size(P, H, W)
     if n == 0:
-        return None, 0
+        return None, 0, None
     ws = torch.empty(n, dtype=torch.uint8, device=dev)
+    order = None
     e = _order_cache.get(dev.index)
     if e is not None:
         ref, ptr, ver, p, stream, ows = e
         src = ref()
         if (src is not None and p == P and ptr == normal.data_ptr() == src.data_ptr()
                 and ver == src._version == normal._version and stream == _stream(dev)):
-            _lib.reuse_point_order_once(ows.data_ptr())
-    return ws, n
+            order = (ows.data_ptr(), P)
+    return ws, n, order
 
 
 def _fwd_workspace(P: int, H: int, W: int, dev: torch.device):
@@ -234,11 +236,11 @@ def bruteforce_diffuse_shader_backward(grad_shading, normal, albedo, envmap, lig
     grad_albedo = torch.empty((P, C), dtype=normal.dtype, device=dev)
     grad_envmap = torch.empty((C, H, W), dtype=normal.dtype, device=dev)
     with torch.cuda.device(dev):
-        ws, nws = _workspace(P, H, W, dev, normal)
+        ws, nws, order = _workspace(P, H, W, dev, normal)
         _lib.call("des_bruteforce_diffuse_shader_backward", P, C, H, W, _ptr(grad_shading),
                   _ptr(normal), _ptr(albedo), _ptr(envmap), _ptr(light2obj_rotmat),
                   _ptr(grad_normal), _ptr(grad_albedo), _ptr(grad_envmap), _ptr(ws), nws,
-                  _stream(dev))
+                  _stream(dev), order=order)
     return [grad_normal, grad_albedo, grad_envmap]
 
 
@@ -276,11 +278,11 @@ def _spec_backward(entry, grad_shading, normal, view_dir, albedo, specular_albed
     _grad_checks(P, C, dev, (("grad_shading", grad_shading),))
     outs = _alloc_grads(P, C, H, W, normal)
     with torch.cuda.device(dev):
-        ws, nws = _workspace(P, H, W, dev, normal)
+        ws, nws, order = _workspace(P, H, W, dev, normal)
         _lib.call(entry, P, C, H, W, _ptr(grad_shading), _ptr(normal), _ptr(view_dir),
                   _ptr(albedo), _ptr(specular_albedo), _ptr(roughness), _ptr(envmap),
                   _ptr(light2obj_rotmat), int(bool(enable_diffuse)), int(bool(enable_specular)),
-                  *(_ptr(o) for o in outs), _ptr(ws), nws, _stream(dev))
+                  *(_ptr(o) for o in outs), _ptr(ws), nws, _stream(dev), order=order)
     return outs
 
 
@@ -362,12 +364,12 @@ def bruteforce_specular_shader_backward2(grad_shading, grad_diffuse_shading, nor
                              ("grad_diffuse_shading", grad_diffuse_shading)))
     outs = _alloc_grads(P, C, H, W, normal)
     with torch.cuda.device(dev):
-        ws, nws = _workspace(P, H, W, dev, normal)
+        ws, nws, order = _workspace(P, H, W, dev, normal)
         _lib.call("des_bruteforce_specular_shader_backward2", P, C, H, W, _ptr(grad_shading),
                   _ptr(grad_diffuse_shading), _ptr(normal), _ptr(view_dir), _ptr(albedo),
                   _ptr(specular_albedo), _ptr(roughness), _ptr(envmap), _ptr(light2obj_rotmat),
                   int(bool(enable_specular)), *(_ptr(o) for o in outs), _ptr(ws), nws,
-                  _stream(dev))
+                  _stream(dev), order=order)
     return outs
 
 
@@ -419,13 +421,13 @@ def bruteforce_specularvisibility_shader_backward2(grad_shading, grad_diffuse_sh
                              visibility)
     outs = _alloc_grads(P, C, H, W, normal)
     with torch.cuda.device(dev):
-        ws, nws = _workspace(P, H, W, dev, normal)
+        ws, nws, order = _workspace(P, H, W, dev, normal)
         _lib.call("des_bruteforce_specularvisibility_shader_backward2", P, C, H, W, NF, NV, NS,
                   _ptr(grad_shading), _ptr(grad_diffuse_shading), _ptr(normal), _ptr(view_dir),
                   _ptr(albedo), _ptr(specular_albedo), _ptr(roughness), _ptr(envmap),
                   _ptr(light2obj_rotmat), _ptr(face_vertex_list), _ptr(nearest_triangle_id),
                   _ptr(barycentric_coord), _ptr(visibility), int(bool(enable_specular)),
-                  *(_ptr(o) for o in outs), _ptr(ws), nws, _stream(dev))
+                  *(_ptr(o) for o in outs), _ptr(ws), nws, _stream(dev), order=order)
     return outs
 
 

@@ -107,7 +107,7 @@ def load() -> ctypes.CDLL:
         L.des_set_tiling_override.restype = _int
         L.des_set_backward_timing_events.argtypes = [_vp, _vp]
         L.des_set_backward_timing_events.restype = _int
-        L.des_reuse_point_order_once.argtypes = [_vp]
+        L.des_reuse_point_order_once.argtypes = [_vp, _i64]
         L.des_reuse_point_order_once.restype = _int
         _lib = L
     return _lib
@@ -119,8 +119,23 @@ def check(rc: int) -> None:
         raise ShaderError(rc, msg or ERROR_NAMES.get(rc, f"error {rc}"))
 
 
-def call(name: str, *args) -> None:
-    check(getattr(load(), name)(*args))
+def call(name: str, *args, order=None) -> None:
+    """Call entry point `name`; with order = (forward workspace pointer, P) the
+    call takes its point order from that workspace (des_reuse_point_order_once).
+    The hand-over is armed right before the call and dropped if the call
+    cannot be made, so it never reaches a later call."""
+    L = load()
+    fn = getattr(L, name)
+    if order is None:
+        check(fn(*args))
+        return
+    check(L.des_reuse_point_order_once(*order))
+    try:
+        rc = fn(*args)
+    except BaseException:
+        L.des_reuse_point_order_once(None, 0)
+        raise
+    check(rc)
 
 
 def workspace_size(P: int, H: int, W: int) -> int:
@@ -131,10 +146,10 @@ def forward_workspace_size(P: int, H: int, W: int) -> int:
     return int(load().des_forward_workspace_size(P, H, W))
 
 
-def reuse_point_order_once(order_workspace_ptr) -> None:
-    """The calling thread's next shading call takes its point order from this
-    forward workspace (see des_reuse_point_order_once)."""
-    check(load().des_reuse_point_order_once(order_workspace_ptr))
+def reuse_point_order_once(order_workspace_ptr, P: int) -> None:
+    """The calling thread's next shading call for P points takes its point order
+    from this forward workspace (see des_reuse_point_order_once)."""
+    check(load().des_reuse_point_order_once(order_workspace_ptr, P))
 
 
 def set_point_order(mode: int) -> None:

@@ -22,9 +22,11 @@ thread_local std::string g_last_error;
 // Point order handed to the calling thread's next call (des_reuse_point_order_once);
 // every call returning through fail() or ok() drops it, the shading calls take it.
 thread_local const void* g_reuse_order = nullptr;
+thread_local int64_t g_reuse_order_P = -1;
 
 int fail(int code, const char* fmt, ...) {
   g_reuse_order = nullptr;
+  g_reuse_order_P = -1;
   char buf[512];
   va_list ap;
   va_start(ap, fmt);
@@ -36,13 +38,16 @@ int fail(int code, const char* fmt, ...) {
 
 int ok() {
   g_reuse_order = nullptr;
+  g_reuse_order_P = -1;
   g_last_error.clear();
   return DES_OK;
 }
 
-const void* take_reuse_order() {
-  const void* r = g_reuse_order;
+// The handed-over order if it was computed for P points, else null.
+const void* take_reuse_order(int64_t P) {
+  const void* r = g_reuse_order_P == P ? g_reuse_order : nullptr;
   g_reuse_order = nullptr;
+  g_reuse_order_P = -1;
   return r;
 }
 
@@ -187,7 +192,7 @@ int launch_error(hipError_t e, const char* what) {
 
 int run_forward(int variant, bool diff, bool spec, des::Problem& pr, void* ws, size_t ws_bytes,
                 hipStream_t stream) {
-  const void* reuse = des_capi::take_reuse_order();
+  const void* reuse = des_capi::take_reuse_order(pr.P);
   if (pr.P == 0) return ok();
   if (reuse != nullptr && use_order(pr.P, pr.T, false)) {
     pr.order = reinterpret_cast<const int32_t*>(static_cast<const char*>(reuse) +
@@ -211,7 +216,7 @@ int run_forward(int variant, bool diff, bool spec, des::Problem& pr, void* ws, s
 
 int run_backward(int variant, bool diff, bool spec, des::Problem& pr, void* ws, size_t ws_bytes,
                  hipStream_t stream) {
-  const void* reuse = des_capi::take_reuse_order();
+  const void* reuse = des_capi::take_reuse_order(pr.P);
   REQUIRE(pr.gE);
   if (pr.P == 0) {
     hipError_t e = des::launch_fill_zero(pr.gE, int64_t(3) * pr.T, stream);
@@ -391,11 +396,14 @@ DES_API size_t des_forward_workspace_size(int64_t P, int32_t H, int32_t W) {
   return des::order_layout(P).total;
 }
 
-DES_API int des_reuse_point_order_once(const void* order_workspace) {
+DES_API int des_reuse_point_order_once(const void* order_workspace, int64_t P) {
   if ((reinterpret_cast<uintptr_t>(order_workspace) & 255) != 0)
     return fail(DES_ERR_INVALID_ARGUMENT, "order workspace must be 256-byte aligned");
+  if (order_workspace != nullptr && P <= 0)
+    return fail(DES_ERR_INVALID_ARGUMENT, "P must be >= 1 (got %lld)", (long long)P);
   const int rc = ok();
   des_capi::g_reuse_order = order_workspace;
+  des_capi::g_reuse_order_P = order_workspace != nullptr ? P : -1;
   return rc;
 }
 

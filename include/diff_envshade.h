@@ -88,15 +88,16 @@ DES_API int des_set_point_order(int32_t mode);
 
 /* Point order shared between calls (no counterpart in the reference).  The
  * calling thread's next call into the library, if it is a shading entry point
- * that would reorder its points, takes the order from `order_workspace`
- * instead of sorting: the workspace of an earlier *_forward* call that sorted
- * (des_forward_workspace_size bytes, 256-byte aligned) with the same P,
- * ordered before this call on the stream.  With the same normals the outputs
- * are bitwise those of a call that sorts by itself; every order is a
+ * for P points that would reorder them, takes the order from
+ * `order_workspace` instead of sorting: the workspace of an earlier
+ * *_forward* call for P points that sorted (des_forward_workspace_size bytes,
+ * 256-byte aligned), ordered before this call on the stream.  A call for
+ * another number of points ignores it.  With the same normals the outputs are
+ * bitwise those of a call that sorts by itself; every order of P points is a
  * permutation, so with other normals they stay correct and move at rounding
  * level only.  The next call consumes the hook whatever it is; NULL clears
  * it. */
-DES_API int des_reuse_point_order_once(const void* order_workspace);
+DES_API int des_reuse_point_order_once(const void* order_workspace, int64_t P);
 
 /* bindings.cpp:6 -> bruteforce_diffuse_shader_forward (bruteforce_shader.cu:2687-2731).
  * Lambert only: shading = sum_t [l.n>0] E a (l.n/pi) dOmega. */

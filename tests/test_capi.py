@@ -283,9 +283,29 @@ def test_sample_validation():
 
 def test_reuse_point_order_hook_validation():
     L = _lib.load()
-    assert L.des_reuse_point_order_once(ctypes.c_void_p(DUMMY + 4)) == -1
+    assert L.des_reuse_point_order_once(ctypes.c_void_p(DUMMY + 4), 10) == -1
     assert "256-byte aligned" in L.des_last_error().decode()
-    assert L.des_reuse_point_order_once(ctypes.c_void_p(DUMMY)) == 0
+    assert L.des_reuse_point_order_once(ctypes.c_void_p(DUMMY), 0) == -1
+    assert L.des_reuse_point_order_once(ctypes.c_void_p(DUMMY), 10) == 0
     # any call consumes the hook: a failing one here, before any launch
     assert _vis_fwd(C=4) == -2
-    assert L.des_reuse_point_order_once(None) == 0
+    assert L.des_reuse_point_order_once(None, 0) == 0
+
+
+def test_call_drops_the_hand_over_when_the_call_cannot_be_made(monkeypatch):
+    # the entry point cannot be called (here: wrong argument count, as a
+    # ctypes conversion error would): the helper disarms before re-raising
+    hooks = []
+
+    class Fake:
+        def des_reuse_point_order_once(self, ptr, P):
+            hooks.append((ptr, P))
+            return 0
+
+        def des_bruteforce_specular_shader_backward2(self, *args):
+            raise ctypes.ArgumentError("argument 1: wrong type")
+
+    monkeypatch.setattr(_lib, "load", lambda: Fake())
+    with pytest.raises(ctypes.ArgumentError):
+        _lib.call("des_bruteforce_specular_shader_backward2", "x", order=(DUMMY, 10))
+    assert hooks == [(DUMMY, 10), (None, 0)]
