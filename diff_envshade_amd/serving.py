@@ -49,7 +49,7 @@ class Request:
 
 
 REDUCE_ROWS = 16  # partial rows per stage-1 group of the env-gradient reduce
-ORDER_MIN_PAIRS = 1 << 24  # sets of this many point-texel pairs are sorted (backward policy)
+ORDER_MIN_PAIRS = 1 << 24  # sets of this many point-texel pairs are sorted (both directions)
 ORDER_BLOCK = 1024  # points per block of the counting sort
 ORDER_KEYS = 1024  # keys (octahedral cells) of the counting sort
 
