@@ -166,3 +166,26 @@ def test_an_order_of_other_normals_is_still_correct(reuse_spy):
     outs = backward((ows.data_ptr(), P - 1))
     for a, b in zip(fresh, outs):
         assert torch.equal(a, b)
+
+
+def test_autograd_backward_takes_the_forward_order(reuse_spy):
+    # through the torch.autograd.Function: the saved normals are the forward's
+    # (same storage and version counter), so the backward takes its order; the
+    # gradients are bitwise those of a backward that sorts by itself
+    import torch
+    from diff_envshade_amd import _diff_envshade as de
+    from diff_envshade_amd.autograd import SpecularVisibilityShader2
+    t = _device(problem(seed=33, n=40000, H=16, W=32, vis="identity"))
+    names = ("normal", "view_dir", "albedo", "specular_albedo", "roughness", "envmap")
+    leaves = {k: t[k].clone().requires_grad_(True) for k in names}
+    sh, ds = SpecularVisibilityShader2.apply(
+        *(leaves[k] for k in names), t["light2obj_rotmat"], t["face_vertex_list"],
+        t["nearest_triangle_id"], t["barycentric_coord"], t["visibility"], True)
+    ((sh * t["grad_shading"]).sum() + (ds * t["grad_diffuse_shading"]).sum()).backward()
+    assert len(reuse_spy) == 1
+    de._order_cache.clear()
+    mat, vis = _vis_args(t)
+    ref = de.bruteforce_specularvisibility_shader_backward2(
+        t["grad_shading"], t["grad_diffuse_shading"], *mat, *vis, True)
+    for k, r in zip(names, ref):
+        assert torch.equal(leaves[k].grad.reshape(r.shape), r), k
