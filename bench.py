@@ -333,7 +333,8 @@ def _queued_avg_ms(fn, n):
     a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     fn()
     torch.cuda.synchronize()
-    torch.cuda._sleep(50_000_000)  # longer than the host needs to enqueue the n calls
+    if hasattr(torch.cuda, "_sleep"):  # a spin longer than the host needs to enqueue the n calls
+        torch.cuda._sleep(50_000_000)
     a.record()
     for _ in range(n):
         fn()
