@@ -831,6 +831,12 @@ def bench_batched(args):
                    "ntc_backward": ntc_b, "graph": True},
         "eager_ms_per_step": ms_eager,
         "achieved_TFLOPs": pairs * (FLOP_FWD_PER_PAIR + FLOP_BWD_PER_PAIR) / (ms * 1e-3) / 1e12,
+        "roofline": {"bound": "mfma", "pipe": "fp32 vector ALU (no MFMA on this path)",
+                     "kernel": "batch forward + backward (whole step)",
+                     "achieved": pairs * (FLOP_FWD_PER_PAIR + FLOP_BWD_PER_PAIR) / (ms * 1e-3) / 1e12,
+                     "peak": PEAK_FP32_FLOPS / 1e12, "unit": "TFLOP/s",
+                     "frac": pairs * (FLOP_FWD_PER_PAIR + FLOP_BWD_PER_PAIR) / (ms * 1e-3)
+                     / PEAK_FP32_FLOPS, "traffic": None},
     }
 
 
@@ -947,7 +953,31 @@ def bench_relight(args):
         "precisions": res,
         "fp32_dropin": {"ms_fwd_bwd_one_map": ms_d, "pairs_per_s": dropin_rate},
         "speedup_vs_fp32_dropin": {k: v["pairs_per_s"] / dropin_rate for k, v in res.items()},
+        "roofline": _relight_roofline(P, E, H * W, res[best]["forward_ms"], best),
     }
+
+
+# MI355X dense MFMA peaks by operand type (no sparsity)
+PEAK_MFMA = {"bf16": 2.5e15, "fp16": 2.5e15, "bf16x3": 2.5e15}
+
+
+def _relight_roofline(P, E, T, fwd_ms, prec):
+    """The relight forward: per 32-texel chunk the weights W (P x 2T, diffuse
+    and specular) are generated on the VALU (65 FP32 operations per pair, as
+    the drop-in forward counts them) and multiplied against the maps (2T x 3E)
+    on MFMA: 2 x P x 2T x 3E FLOP per product, three products per bf16x3
+    term pair."""
+    mfma = 2.0 * P * (2 * T) * (3 * E) * (3 if prec == "bf16x3" else 1)
+    valu = float(P) * T * FLOP_FWD_PER_PAIR
+    t = fwd_ms * 1e-3
+    return {"bound": "mfma", "kernel": f"relight_forward_kernel<{prec}>",
+            "achieved": mfma / t / 1e12, "peak": PEAK_MFMA[prec] / 1e12, "unit": "TFLOP/s",
+            "frac": mfma / t / PEAK_MFMA[prec], "traffic": None,
+            "valu_weight_generation": {"achieved": valu / t / 1e12,
+                                       "peak": PEAK_FP32_FLOPS / 1e12,
+                                       "frac": valu / t / PEAK_FP32_FLOPS},
+            "note": "forward only; weight generation on the VALU and the MFMA products "
+                    "run in the same kernel"}
 
 
 def bench_serving(args):
