@@ -668,7 +668,7 @@ __global__ __launch_bounds__(1024) void shade_forward_kernel(Problem pr, Tiling 
   forward_tile<V, DIFF, SPEC>(pr, tl, blockIdx.x, pr.order);
 }
 
-// Three waves per SIMD (<= 168 VGPRs): the measured best for this kernel.
+// Four waves per SIMD (<= 128 VGPRs): the measured best for this kernel.
 template <int V, bool DIFF, bool SPEC>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void shade_backward_kernel(
     Problem pr, Tiling tl) {
