@@ -106,7 +106,7 @@ def main() -> int:
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the shader has no "
-              "multi-GPU path in this build", file=sys.stderr)
+              "multi-GPU path (one process shades on one GPU)", file=sys.stderr)
         return 2
     if args.workload != "dropin":
         line = {"batched": bench_batched, "relight": bench_relight,
