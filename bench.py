@@ -633,11 +633,27 @@ def bench_mesh_vis(args):
                      "frac": achieved / PEAK_FP32_FLOPS, "flop_per_pair": FLOP_BWD_PER_PAIR,
                      "traffic": None},
     }
-    if not args.no_accuracy:
+    if not (args.no_accuracy and args.no_cpu_baseline):
         O = _oracle()
         sh, ds, g = out
-        rsh, rds = O.forward("vis", d)
-        rg = O.backward("vis", d, d["grad_shading"], d["grad_diffuse_shading"])
+        runs = []
+        for _ in range(1 if args.no_cpu_baseline else 6):
+            a = time.perf_counter()
+            rsh, rds = O.forward("vis", d)
+            rg = O.backward("vis", d, d["grad_shading"], d["grad_diffuse_shading"])
+            runs.append(time.perf_counter() - a)
+        if not args.no_cpu_baseline:
+            timed = sorted(runs[1:])
+            med = timed[len(timed) // 2]
+            line["cpu_baseline"] = {"value": P / med, "unit": UNIT, "cores": O.max_threads(),
+                                    "cpu_model": _cpu_model(),
+                                    "omp_proc_bind": os.environ.get("OMP_PROC_BIND"),
+                                    "kind": "port",
+                                    "sample": f"the full workload ({P} points x {T} texels, "
+                                              "forward+backward, FP64 oracle, OpenMP), median "
+                                              f"of {len(timed)} runs after 1 warm-up",
+                                    "seconds_per_step": med}
+    if not args.no_accuracy:
         gn, gv, ga, gs, gr, gE = [x.cpu().numpy() for x in g]
         line["err_vs_fp64"] = {
             "shading": _errors(sh.cpu().numpy(), rsh),
