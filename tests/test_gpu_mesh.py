@@ -1,7 +1,8 @@
 """Mesh utilities on the GPU against the FP64 oracle (oracle/mesh_oracle.c).
 
 Both sides evaluate the reference's closest-point geometry (etc.cu:205-393) in
-FP64 with the same operations, each correctly rounded and none fused, so the
+FP64 as the same sequence of IEEE operations (adds, multiplies, the explicit
+fused multiply-adds, one correctly rounded division per face), so the
 outputs agree exactly: triangle ids bit for bit, and barycentrics and
 gradients as the float32 rounding of the oracle's FP64 values.  Exact
 agreement matters for the triangle id: a query whose closest point lies on an
@@ -191,3 +192,25 @@ def test_nearest_points_feed_the_visibility_shader():
     rsh, rds = O.forward("vis", p, True, True)
     assert relerr(sh.cpu().numpy(), rsh) < TOL
     assert relerr(ds.cpu().numpy(), rds) < TOL
+
+
+def test_nearest_forward_on_malformed_adjacency():
+    # adjacency lists that do not match the faces (random faces, repeats within
+    # and across lists, out-of-range face ids, faces with out-of-range
+    # vertices) on a jittered, non-convex surface: the GPU still picks exactly
+    # the oracle's face -- the first candidate at the smallest distance
+    rng = np.random.default_rng(12)
+    d = WL.mesh_queries(seed=12, subdiv=3, n=3000, K=12, jitter=0.2, spread=0.3)
+    NV, NF = d["vertex_pos"].shape[1], d["face_vertex_list"].shape[1]
+    count = rng.integers(0, 9, size=NV).astype(np.int32)
+    adj = rng.integers(-3, NF + 3, size=int(count.sum())).astype(np.int32)
+    adj[::7] = adj[::7] // 2            # repeats
+    d["adjacency_head"] = np.stack([count, np.cumsum(count).astype(np.int32)])
+    d["adjacency_list"] = adj
+    fvl = d["face_vertex_list"].copy()
+    fvl[1, ::13] = NV + 5               # faces with an out-of-range vertex
+    d["face_vertex_list"] = fvl
+    d["idxs"][::11, 3] = -1
+    d["idxs"][::17, 5] = NV + 1
+    tri, _ = check_nearest(d)
+    assert (tri == -1).sum() < tri.size // 2
