@@ -12,6 +12,9 @@ zjumsj/diff_envshade):
     bruteforce_shader.cu:2457);
   * a None upstream gradient is treated as zeros, and upstream gradients are
     made contiguous before the backward binding is called.
+
+NearestMeshPoints and UVofTriangle wrap the mesh utilities the same way: their
+barycentric outputs are differentiable in the query points.
 """
 from __future__ import annotations
 
@@ -162,3 +165,55 @@ def specular_visibility_shade2(normal, view_dir, albedo, specular_albedo, roughn
                                            envmap, light2obj_rotmat, face_vertex_list,
                                            nearest_triangle_id, barycentric_coord, visibility,
                                            enable_specular)
+
+
+# ---- mesh utilities (etc.cu:707-920): gradients flow to the query points only ----
+
+class NearestMeshPoints(torch.autograd.Function):
+    """(nearest_triangle_id, barycentric_coord) of each query point
+    (get_nearest_mesh_points_forward); the barycentrics are differentiable in
+    query_pos (3 x P) through get_nearest_mesh_points_backward, with the chosen
+    face held fixed.  The triangle ids are integers and carry no gradient."""
+
+    @staticmethod
+    def forward(ctx, adjacency_head, adjacency_list, face_vertex_list, vertex_pos, query_pos,
+                idxs):
+        tri, bary = _C.get_nearest_mesh_points_forward(adjacency_head, adjacency_list,
+                                                       face_vertex_list, vertex_pos, query_pos,
+                                                       idxs)
+        ctx.save_for_backward(face_vertex_list, vertex_pos, query_pos, tri)
+        ctx.mark_non_differentiable(tri)
+        return tri, bary
+
+    @staticmethod
+    def backward(ctx, grad_tri, grad_bary):
+        fvl, vpos, query, tri = ctx.saved_tensors
+        gq = _C.get_nearest_mesh_points_backward(fvl, vpos, query, tri, _g(grad_bary, query))
+        return None, None, None, None, gq, None
+
+
+class UVofTriangle(torch.autograd.Function):
+    """Barycentric coordinates (P x 3) of the closest point of each triangle
+    (P x 3 x 3) to its query point (P x 3), get_uv_of_triangle_forward;
+    differentiable in query_pos through get_uv_of_triangle_backward."""
+
+    @staticmethod
+    def forward(ctx, triangles, query_pos):
+        bary = _C.get_uv_of_triangle_forward(triangles, query_pos)
+        ctx.save_for_backward(triangles, query_pos)
+        return bary
+
+    @staticmethod
+    def backward(ctx, grad_bary):
+        triangles, query = ctx.saved_tensors
+        return None, _C.get_uv_of_triangle_backward(triangles, query, _g(grad_bary, query))
+
+
+def nearest_mesh_points(adjacency_head, adjacency_list, face_vertex_list, vertex_pos, query_pos,
+                        idxs):
+    return NearestMeshPoints.apply(adjacency_head, adjacency_list, face_vertex_list, vertex_pos,
+                                   query_pos, idxs)
+
+
+def uv_of_triangle(triangles, query_pos):
+    return UVofTriangle.apply(triangles, query_pos)

@@ -214,3 +214,26 @@ def test_nearest_forward_on_malformed_adjacency():
     d["idxs"][::17, 5] = NV + 1
     tri, _ = check_nearest(d)
     assert (tri == -1).sum() < tri.size // 2
+
+
+def test_mesh_autograd_functions():
+    # the autograd wrappers give the backward bindings' gradients
+    from diff_envshade_amd.autograd import nearest_mesh_points, uv_of_triangle
+    d = WL.mesh_queries(seed=5, subdiv=3, n=2000, K=8)
+    t = cuda(d)
+    q = t["query_pos"].clone().requires_grad_(True)
+    tri, bary = nearest_mesh_points(t["adjacency_head"], t["adjacency_list"],
+                                    t["face_vertex_list"], t["vertex_pos"], q, t["idxs"])
+    g = torch.randn_like(bary)
+    (bary * g).sum().backward()
+    want = de.get_nearest_mesh_points_backward(t["face_vertex_list"], t["vertex_pos"],
+                                               t["query_pos"], tri, g)
+    assert torch.equal(q.grad, want)
+    assert not tri.requires_grad
+    rng = np.random.default_rng(6)
+    tris = torch.from_numpy(rng.normal(size=(500, 3, 3)).astype(np.float32)).to(DEV)
+    qp = torch.from_numpy(rng.normal(size=(500, 3)).astype(np.float32)).to(DEV).requires_grad_(True)
+    b = uv_of_triangle(tris, qp)
+    g2 = torch.randn_like(b)
+    (b * g2).sum().backward()
+    assert torch.equal(qp.grad, de.get_uv_of_triangle_backward(tris, qp.detach(), g2))
