@@ -219,3 +219,32 @@ def test_automatic_order_policy_in_a_batch():
                 assert np.array_equal(a.reshape(s.shape), s)
     finally:
         _lib.set_tiling_override(0, 0)
+
+
+@pytest.mark.parametrize("order", [-1, 1])
+def test_envgrad_reduce_of_single_set_matches_batch_at_many_groups(order):
+    # the single-set backward and the batch reduce the env-map partials with
+    # the same two stages in different launches (one set vs a descriptor table);
+    # both add in the same order, so every output agrees bit for bit.  137 and
+    # 42 groups of 16 tiles: a second round of eight loads per lane in the final
+    # sum (lanes q < 9 of 16), partial rounds, and partial column blocks (96
+    # and 768 columns)
+    from diff_envshade_amd import _lib
+    from diff_envshade_amd.batch import ShadeBatch
+    nps, devs = _sets([(140000, 4, 8), (43000, 8, 32)], seed=50)
+    try:
+        _lib.set_point_order(order)
+        b = ShadeBatch("vis", enable_specular=True, ntc_forward=4, ntc_backward=4)
+        outs = b.prepare(devs, with_backward=True)
+        b.forward()
+        b.backward()
+        torch.cuda.synchronize()
+        _lib.set_tiling_override(4, 4)
+        for d, o in zip(nps, outs):
+            got = _np(o)
+            single = gpu_run("vis", d)
+            for a, s in zip(got[0] + got[1], single[0] + single[1]):
+                assert np.array_equal(a.reshape(s.shape), s)
+    finally:
+        _lib.set_tiling_override(0, 0)
+        _lib.set_point_order(0)
