@@ -55,6 +55,27 @@ def problem(seed=0, n=300, H=4, W=8, rotation=True, vis="none", p_vis=0.7, n_bri
     return d
 
 
+def stress_problem(seed, n=1500, H=8, W=16):
+    """Sharp and maximal lobes, grazing and back-facing views, bright texels:
+    roughness in {0, 0.01, 0.03, 0.08, 0.3, 1} (alpha^2 down to 1e-8, so D
+    peaks near 1 / (pi alpha^2)); view directions a third uniform on the
+    sphere (half of them back-facing), a third grazing (n.v ~ 1e-3) and a
+    third along the normal; three texels 50x brighter."""
+    d = problem(seed=seed, n=n, H=H, W=W, vis="identity", n_bright=3)
+    rng = np.random.default_rng(seed + 100)
+    d["roughness"] = rng.choice([0.0, 0.01, 0.03, 0.08, 0.3, 1.0], size=(n, 1)).astype(np.float32)
+    nrm = d["normal"].astype(np.float64)
+    t = np.cross(nrm, rng.normal(size=(n, 3)))
+    t /= np.linalg.norm(t, axis=1, keepdims=True)
+    u = rng.normal(size=(n, 3))
+    u /= np.linalg.norm(u, axis=1, keepdims=True)
+    graze = t + 1e-3 * nrm
+    graze /= np.linalg.norm(graze, axis=1, keepdims=True)
+    kind = (np.arange(n) % 3)[:, None]
+    d["view_dir"] = np.where(kind == 0, u, np.where(kind == 1, graze, nrm)).astype(np.float32)
+    return d
+
+
 def oracle_run(variant, d, diff=True, spec=True):
     sh, ds = O.forward(variant, d, diff, spec)
     fwd = [sh] if ds is None else [sh, ds]

@@ -108,3 +108,33 @@ def test_relight_envgrad_deterministic():
     a = relight_backward_envmap(*args).cpu().numpy()
     b = relight_backward_envmap(*args).cpu().numpy()
     assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("precision", ["bf16x3", "bf16"])
+def test_relight_sharp_lobes_grazing_views(precision):
+    # the shader's stress inputs (roughness down to 0, grazing and back-facing
+    # views, bright texels) under four maps; fp16 is left out: its 65504 range
+    # does not hold the GGX weights of lobes this sharp
+    from _helpers import stress_problem
+    from diff_envshade_amd.relight import relight_backward_envmap, relight_forward
+    H, W, n, nE = 8, 16, 1500, 4
+    d = stress_problem(seed=11, n=n, H=H, W=W)
+    rng = np.random.default_rng(12)
+    envs = np.stack([WL.make_envmap(rng, H, W, 1.0, n_bright=2) for _ in range(nE)])
+    gs = rng.normal(size=(n, nE, 3)).astype(np.float32)
+    gd = rng.normal(size=(n, nE, 3)).astype(np.float32)
+    t = gpu_inputs(d)
+    E = torch.from_numpy(envs).cuda()
+    sh, ds = relight_forward(t["normal"], t["view_dir"], t["albedo"], t["specular_albedo"],
+                             t["roughness"], E, t["light2obj_rotmat"], t["visibility"], True,
+                             precision)
+    gE = relight_backward_envmap(t["normal"], t["view_dir"], t["albedo"], t["specular_albedo"],
+                                 t["roughness"], H, W, t["light2obj_rotmat"], t["visibility"],
+                                 torch.from_numpy(gs).cuda(), torch.from_numpy(gd).cuda(), True,
+                                 precision)
+    torch.cuda.synchronize()
+    rsh, rds, rgE = per_map_oracle(d, envs, gs, gd)
+    errs = {"shading": relerr(sh.cpu().numpy(), rsh), "diffuse_shading": relerr(ds.cpu().numpy(), rds),
+            "grad_envmap": relerr(gE.cpu().numpy(), rgE)}
+    print(precision, errs)
+    assert max(errs.values()) <= TOLS[precision], errs
