@@ -17,7 +17,7 @@ reproducible.
 The mesh utilities that produce the shader's mesh inputs (nearest mesh
 points, closest point on a given triangle, visibility bitfield decoding;
 bindings.cpp:27-33, etc.cu) and the env-map sampling and blur beside it
-(bindings.cpp:20, 22) follow the same conventions.  The other etc.cu bindings
+(bindings.cpp:21, 23) follow the same conventions.  The other etc.cu bindings
 (camera rays, rotation conversion) are outside this library and raise
 NotImplementedError.
 """
@@ -559,7 +559,7 @@ def extract_bitfield_forward(bitfield, N_bit, i_start, i_end):
     return out
 
 
-# ---- env-map preprocessing (bindings.cpp:20, 22, etc.cu) --------------------------------
+# ---- env-map preprocessing (bindings.cpp:21, 23, etc.cu) --------------------------------
 
 def _envmap_shape(envmap):
     _check_float(envmap, "envmap")
@@ -570,7 +570,7 @@ def _envmap_shape(envmap):
 
 def sample_envmap_forward(query_point, envmap):
     """Bilinear lookup of envmap (C x H x W) along each direction of
-    query_point (3 x P, normalised here), etc.cu:1147-1178: returns C x P (the
+    query_point (3 x P, normalised here), etc.cu:1145-1178: returns C x P (the
     reference allocates 3 x P whatever C is)."""
     C, H, W = _envmap_shape(envmap)
     _check_float(query_point, "query_point")
@@ -588,7 +588,7 @@ def sample_envmap_forward(query_point, envmap):
 
 def blur_envmap_forward(envmap, tar_H, tar_W, N_samples, alpha):
     """GGX blur of envmap (C x H x W, C = 3 or 4) onto a tar_H x tar_W map with
-    N_samples directions per texel, etc.cu:1103-1145: returns C x tar_H x tar_W."""
+    N_samples directions per texel, etc.cu:1103-1143: returns C x tar_H x tar_W."""
     C, H, W = _envmap_shape(envmap)
     dev = envmap.device
     tar_H, tar_W, N_samples = int(tar_H), int(tar_W), int(N_samples)
@@ -599,7 +599,7 @@ def blur_envmap_forward(envmap, tar_H, tar_W, N_samples, alpha):
     return out
 
 
-# ---- reference bindings outside this library (bindings.cpp:24, 31, etc.cu) -------------
+# ---- reference bindings outside this library (bindings.cpp:25, 32, etc.cu) -------------
 
 def _not_here(name):
     def fn(*args, **kwargs):

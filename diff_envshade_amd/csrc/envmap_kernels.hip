@@ -94,7 +94,7 @@ __device__ __forceinline__ double blend(const Taps& t, const float* __restrict__
          t.ky * ((1.0 - t.kx) * v10 + t.kx * v11);
 }
 
-// SampleEnvmap_Forward (etc.cu:689-705): one thread per query point
+// SampleEnvmap_Forward (etc.cu:687-705): one thread per query point
 // (query_point 3 x P, normalised here), output C x P.
 __global__ __launch_bounds__(256) void sample_envmap_kernel(int64_t P, int C, int H, int W,
                                                             const float* __restrict__ q,
@@ -109,7 +109,7 @@ __global__ __launch_bounds__(256) void sample_envmap_kernel(int64_t P, int C, in
   for (int c = 0; c < C; ++c) out[c * P + p] = (float)blend(t, env + c * plane);
 }
 
-// BlurEnvmap_Forward (etc.cu:607-687): WPT waves per target texel (1 or all 4
+// BlurEnvmap_Forward (etc.cu:603-685): WPT waves per target texel (1 or all 4
 // of the block).  The texel's lanes take samples l, l + 64 WPT, ...; each adds
 // its GGX-distributed lookups in FP64, each wave sums its lanes in a fixed tree
 // order and the block sums its waves in order, so the result is reproducible

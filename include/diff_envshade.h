@@ -4,7 +4,7 @@
  * (bindings.cpp:6-19, prototypes gsface_shader.h:3-118, host wrappers
  * bruteforce_shader.cu:2687-3453), for the five mesh utilities that produce
  * its mesh inputs (bindings.cpp:27-30, 33; etc.cu) and for the two env-map
- * utilities beside it (bindings.cpp:20, 22; etc.cu), declared at the end.
+ * utilities beside it (bindings.cpp:21, 23; etc.cu), declared at the end.
  *
  * Conventions (all entry points):
  *   - every pointer is a device pointer to contiguous row-major data on the
@@ -391,7 +391,7 @@ DES_API int des_extract_bitfield_forward(const int32_t* bitfield, int32_t N_slot
  * FP64 directions, angles, bilinear weights and sums; row 0 is the top (+y),
  * the seam at phi = 0 wraps (the reference's config.h). */
 
-/* sample_envmap_forward (etc.cu:1147-1178, kernel etc.cu:689-705): bilinear
+/* sample_envmap_forward (etc.cu:1145-1178, kernel etc.cu:687-705): bilinear
  * lookup of envmap (C x H x W) along each query direction (query_point 3 x P,
  * normalised here).  output is C x P (the reference allocates 3 x P whatever
  * C is, etc.cu:1164); a zero-length query samples NaN. */
@@ -399,7 +399,7 @@ DES_API int des_sample_envmap_forward(int64_t P, int32_t C, int32_t H, int32_t W
                                       const float* query_point, const float* envmap, float* output,
                                       void* stream);
 
-/* blur_envmap_forward (etc.cu:1103-1145, kernel etc.cu:607-687): each texel of
+/* blur_envmap_forward (etc.cu:1103-1143, kernel etc.cu:603-685): each texel of
  * a tar_H x tar_W map is the mean of N_samples bilinear lookups along
  * GGX-distributed directions (roughness parameter alpha) around its centre
  * direction; C must be 3 or 4 (DES_ERR_UNSUPPORTED_CHANNELS, "Channel number

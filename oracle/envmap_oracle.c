@@ -3,9 +3,9 @@
  * bench.py's CPU baseline through oracle/oracle.py, never by the product.
  *
  *   bilinear lookup of a direction, seam wrapped      SampleEnvmap etc.cu:527-586
- *   sample_envmap_forward                             etc.cu:689-705, 1147-1178
+ *   sample_envmap_forward                             etc.cu:687-705, 1145-1178
  *   GGX (Trowbridge-Reitz) direction sampling         etc.cu:588-605
- *   blur_envmap_forward                               etc.cu:607-687, 1103-1145
+ *   blur_envmap_forward                               etc.cu:603-685, 1103-1143
  *
  * Conventions of config.h (LEFT_TOP_AS_ORIGIN, FIX_ENVMAP_SEAM): row 0 is the
  * top (+y), the seam at phi = 0 wraps.
@@ -87,7 +87,7 @@ static void lookup(double x, double y, double z, int H, int W, int C, const floa
   }
 }
 
-/* SampleEnvmap_Forward (etc.cu:689-705): query_point 3 x P (normalised here),
+/* SampleEnvmap_Forward (etc.cu:687-705): query_point 3 x P (normalised here),
  * output C x P. */
 void des_oracle_sample_envmap(int64_t P, int C, int H, int W, const double* q, const float* env,
                               double* out) {
@@ -113,7 +113,7 @@ void des_oracle_ggx_sample(double r0, double r1, double alpha, double* l) {
   l[2] = cos_t;
 }
 
-/* BlurEnvmap_Forward (etc.cu:607-687): for each texel of a tar_H x tar_W map,
+/* BlurEnvmap_Forward (etc.cu:603-685): for each texel of a tar_H x tar_W map,
  * the mean of N bilinear lookups along GGX-distributed directions around the
  * texel's centre direction.  Output C x tar_H x tar_W. */
 void des_oracle_blur_envmap(int C, int H, int W, int tar_H, int tar_W, int N, double alpha,

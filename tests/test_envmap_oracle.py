@@ -9,7 +9,7 @@ what the mathematics fixes, on the CPU:
     the texel, the seam interpolates between the last and first column, the
     poles clamp to the first / last row, the result is linear in the map and
     independent of the query's length;
-  * the blur (etc.cu:607-687): alpha = 0 at the source size returns the map, a
+  * the blur (etc.cu:603-685): alpha = 0 at the source size returns the map, a
     constant map stays constant, and a large-sample blur converges to the
     GGX-weighted mean of the lookup computed by quadrature over the sphere.
 """
