@@ -14,7 +14,9 @@ of the drop-in visibility shader (bruteforce_specularvisibility_shader_forward2
 operand precision:
 
   "bf16"   : weights and radiance rounded to bf16, one MFMA per product
-  "fp16"   : the same in fp16 (needs weights below 65504: roughness >~ 0.07)
+  "fp16"   : the same in fp16; with specular on, roughness must be at least
+             FP16_MIN_ROUGHNESS (0.0558) so every GGX weight fits fp16's 65504
+             (ValueError otherwise)
   "bf16x3" : hi + lo bf16 split of both operands, three MFMAs per product
 
 Accumulation is fp32 on the matrix cores; the cross-workgroup env-map-gradient
@@ -23,12 +25,31 @@ sum is FP64 in a fixed order.
 from __future__ import annotations
 
 import ctypes
+import math
 
 import torch
 
 from . import _lib
 
 PRECISIONS = {"bf16": 0, "fp16": 1, "bf16x3": 2}
+
+# The GGX weight F G D / 4 of a pair is at most 2 / (pi r^4): F <= 1, the G
+# term n.l / (n.l (1-k) + k) * 1/4 / (n.v (1-k) + k) <= 2 because
+# k = (r+1)^2 / 8 >= 1/8, and D <= 1 / (pi alpha^2) with alpha^2 = r^4 because
+# tD = alpha^2 (n.h)^2 + 1 - (n.h)^2 >= alpha^2.  fp16 holds |x| <= 65504, so
+# every weight fits from this roughness up.
+FP16_MIN_ROUGHNESS = (2.0 / (math.pi * 65504.0)) ** 0.25
+
+
+def _check_range(precision, enable_specular, roughness):
+    if precision not in PRECISIONS:
+        raise ValueError(f"precision must be one of {sorted(PRECISIONS)} (got {precision!r})")
+    if precision == "fp16" and enable_specular and roughness.numel() > 0:
+        rmin = float(roughness.min())
+        if not rmin >= FP16_MIN_ROUGHNESS:
+            raise ValueError(
+                f"fp16 cannot hold the GGX weights of roughness {rmin:.4g}: it needs roughness "
+                f">= {FP16_MIN_ROUGHNESS:.4f}; use precision='bf16' or 'bf16x3'")
 
 _vp, _i32, _i64, _int, _sz = (ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_int,
                               ctypes.c_size_t)
@@ -76,6 +97,7 @@ def relight_forward(normal, view_dir, albedo, specular_albedo, roughness, envmap
                  ("envmaps", envmaps), ("light2obj_rotmat", light2obj_rotmat)):
         _check(n, t)
     _check("visibility", visibility, torch.int32)
+    _check_range(precision, enable_specular, roughness)
     P = normal.shape[0]
     E, C, H, W = envmaps.shape
     if C != 3:
@@ -106,6 +128,7 @@ def relight_backward_envmap(normal, view_dir, albedo, specular_albedo, roughness
                  ("grad_diffuse_shading", grad_diffuse_shading)):
         _check(n, t)
     _check("visibility", visibility, torch.int32)
+    _check_range(precision, enable_specular, roughness)
     P, E, _ = grad_shading.shape
     dev = normal.device
     gE = torch.empty((E, 3, H, W), dtype=torch.float32, device=dev)

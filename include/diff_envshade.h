@@ -309,7 +309,10 @@ DES_API int des_batch_sort(void* workspace, void* stream);
  * des_relight_backward_envmap gives grad_envmaps (E x 3 x H x W) from the
  * upstream gradients (P x E x 3 each); the cross-block sum is FP64 in fixed
  * order.  precision: DES_PREC_BF16, DES_PREC_FP16, or DES_PREC_BF16X3 (hi+lo
- * split of both operands, three MFMAs per product). */
+ * split of both operands, three MFMAs per product).  fp16 holds weights up to
+ * 65504 and a GGX weight reaches 2 / (pi r^4), so with specular on fp16 needs
+ * roughness >= 0.0558; these entry points do not check it (a sharper lobe
+ * gives inf), the Python calls in diff_envshade_amd/relight.py refuse it. */
 enum { DES_PREC_BF16 = 0, DES_PREC_FP16 = 1, DES_PREC_BF16X3 = 2 };
 
 DES_API size_t des_relight_workspace_size(int64_t P, int32_t E, int32_t H, int32_t W,

@@ -138,3 +138,42 @@ def test_relight_sharp_lobes_grazing_views(precision):
             "grad_envmap": relerr(gE.cpu().numpy(), rgE)}
     print(precision, errs)
     assert max(errs.values()) <= TOLS[precision], errs
+
+
+def test_fp16_refuses_roughness_whose_weights_it_cannot_hold():
+    # with specular on, fp16 needs roughness >= FP16_MIN_ROUGHNESS (the GGX
+    # weight is at most 2 / (pi r^4)); bf16 / bf16x3, fp16 without specular
+    # and fp16 at the bound itself run
+    from diff_envshade_amd.relight import (FP16_MIN_ROUGHNESS, relight_backward_envmap,
+                                           relight_forward)
+    H, W = 8, 16
+    d, envs, gs, gd = relight_problem(3, n=300, E=2, H=H, W=W)
+    d["roughness"][7] = 0.03
+    t = gpu_inputs(d)
+    E = torch.from_numpy(envs).cuda()
+    G = [torch.from_numpy(gs).cuda(), torch.from_numpy(gd).cuda()]
+    mat = [t["normal"], t["view_dir"], t["albedo"], t["specular_albedo"], t["roughness"]]
+
+    def fwd(prec, spec=True):
+        return relight_forward(*mat, E, t["light2obj_rotmat"], t["visibility"], spec, prec)
+
+    def bwd(prec, spec=True):
+        return relight_backward_envmap(*mat, H, W, t["light2obj_rotmat"], t["visibility"], *G,
+                                       spec, prec)
+
+    for call in (fwd, bwd):
+        with pytest.raises(ValueError, match="fp16 cannot hold"):
+            call("fp16")
+        with pytest.raises(ValueError, match="precision must be one of"):
+            call("fp8")
+        for ok in (lambda: call("bf16"), lambda: call("bf16x3"), lambda: call("fp16", False)):
+            out = ok()
+            torch.cuda.synchronize()
+            for x in (out if isinstance(out, tuple) else (out,)):
+                assert torch.isfinite(x).all()
+    t["roughness"][7] = FP16_MIN_ROUGHNESS
+    mat[4] = t["roughness"]
+    sh, ds = fwd("fp16")
+    gE = bwd("fp16")
+    torch.cuda.synchronize()
+    assert torch.isfinite(sh).all() and torch.isfinite(ds).all() and torch.isfinite(gE).all()
