@@ -149,10 +149,13 @@ class ShadeBatch:
                 raise RuntimeError("batch exceeds the workspace the captured graph was built on")
             self.workspace = torch.empty(max(ws.value, 256), dtype=torch.uint8,
                                          device=self.device)
+        # the previous upload must have consumed the staging bytes before they are
+        # rewritten, and before the old pinned block goes back to torch's cache
+        # (which cannot see that HIP still reads it: the pointer went through ctypes)
+        if self._staging_ready is not None:
+            self._staging_ready.synchronize()
         if self.staging is None or self.staging.numel() < st.value:
             self.staging = torch.empty(max(st.value, 256), dtype=torch.uint8, pin_memory=True)
-        elif self._staging_ready is not None:
-            self._staging_ready.synchronize()  # previous upload has consumed the staging bytes
         stream = torch.cuda.current_stream(self.device)
         _lib.check(self.L.des_batch_prepare(
             self.vid, int(self.enable_diffuse), int(self.enable_specular), arr, n,

@@ -10,7 +10,9 @@
 #include <cstdio>
 #include <cstddef>
 #include <cstring>
+#include <mutex>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/diff_envshade.h"
@@ -368,6 +370,7 @@ des::BatchLaunch batch_launch(void* ws, int variant, bool diff, bool spec, bool 
   bl.hdr = static_cast<const des::BatchHeader*>(ws);
   bl.hdr_mut = static_cast<des::BatchHeader*>(ws);
   bl.sets = reinterpret_cast<const des::BatchSet*>(static_cast<char*>(ws) + kHeaderBytes);
+  bl.sig = des::batch_signature(variant, diff, spec, ntc);
   bl.ntc = ntc;
   // table of one super-chunk, or the cross-chunk reduction scratch
   const des::Tiling tl = tiling_for(ntc, des::kMaxSuperChunk, !fwd);
@@ -376,6 +379,46 @@ des::BatchLaunch batch_launch(void* ws, int variant, bool diff, bool spec, bool 
   bl.grid = unsigned(num_cus() * per_cu);
   bl.reduce_grid = unsigned(num_cus() * 4);
   return bl;
+}
+
+// What des_batch_prepare recorded for a workspace, so that a launch with
+// another variant, other branch flags or another launch shape is rejected
+// before it runs (its kernels would read pointers the preparation did not
+// check, or deal the texel groups to waves that do not exist).
+struct PreparedBatch {
+  int variant;
+  bool diff, spec, with_backward;
+  int ntc_fwd, ntc_bwd;
+};
+std::mutex g_batch_mu;
+std::unordered_map<const void*, PreparedBatch> g_batches;
+
+void record_batch(const void* ws, const PreparedBatch& b) {
+  std::lock_guard<std::mutex> lk(g_batch_mu);
+  g_batches[ws] = b;
+}
+
+int check_prepared(const void* ws, int variant, bool diff, bool spec, bool fwd, int ntc) {
+  PreparedBatch b;
+  {
+    std::lock_guard<std::mutex> lk(g_batch_mu);
+    auto it = g_batches.find(ws);
+    if (it == g_batches.end())
+      return fail(DES_ERR_INVALID_ARGUMENT, "workspace was not prepared by des_batch_prepare");
+    b = it->second;
+  }
+  if (b.variant != variant || b.diff != diff || b.spec != spec)
+    return fail(DES_ERR_INVALID_ARGUMENT,
+                "batch was prepared for variant %d (diffuse %d, specular %d), launched as "
+                "variant %d (diffuse %d, specular %d)",
+                b.variant, int(b.diff), int(b.spec), variant, int(diff), int(spec));
+  const int want = fwd ? b.ntc_fwd : b.ntc_bwd;
+  if (want != ntc)
+    return fail(DES_ERR_INVALID_ARGUMENT, "batch was prepared with %s = %d, launched with %d",
+                fwd ? "ntc_forward" : "ntc_backward", want, ntc);
+  if (!fwd && !b.with_backward)
+    return fail(DES_ERR_INVALID_ARGUMENT, "batch was prepared without backward buffers");
+  return DES_OK;
 }
 
 }  // namespace
@@ -736,6 +779,8 @@ DES_API int des_batch_prepare(int32_t variant, int enable_diffuse, int enable_sp
   hdr->n_sets = n_sets;
   hdr->variant = variant;
   hdr->has_backward = with_backward ? 1 : 0;
+  hdr->sig_fwd = des::batch_signature(variant, diff, spec, ntc_forward);
+  hdr->sig_bwd = des::batch_signature(variant, diff, spec, ntc_backward);
   hdr->total_tiles = tile0;
   hdr->total_r1 = r1;
   hdr->total_r2 = r2;
@@ -746,6 +791,8 @@ DES_API int des_batch_prepare(int32_t variant, int enable_diffuse, int enable_sp
   hipError_t e = hipMemcpyAsync(workspace, staging, L.table_bytes, hipMemcpyHostToDevice,
                                 static_cast<hipStream_t>(stream));
   if (e != hipSuccess) return launch_error(e, "descriptor upload failed");
+  record_batch(workspace, PreparedBatch{variant, diff, spec, with_backward != 0, ntc_forward,
+                                        ntc_backward});
   e = des::launch_point_order_jobs(&reinterpret_cast<des::BatchHeader*>(ws)->sort_jobs, num_cus(),
                                    static_cast<hipStream_t>(stream));
   if (e != hipSuccess) return launch_error(e, "point order failed");
@@ -754,6 +801,11 @@ DES_API int des_batch_prepare(int32_t variant, int enable_diffuse, int enable_sp
 
 DES_API int des_batch_sort(void* workspace, void* stream) {
   REQUIRE(workspace);
+  {
+    std::lock_guard<std::mutex> lk(g_batch_mu);
+    if (g_batches.find(workspace) == g_batches.end())
+      return fail(DES_ERR_INVALID_ARGUMENT, "workspace was not prepared by des_batch_prepare");
+  }
   hipError_t e = des::launch_point_order_jobs(
       &static_cast<des::BatchHeader*>(workspace)->sort_jobs, num_cus(),
       static_cast<hipStream_t>(stream));
@@ -769,6 +821,7 @@ DES_API int des_batch_forward(int32_t variant, int enable_diffuse, int enable_sp
   REQUIRE(workspace);
   const bool diff = variant == des::kDiffuse || variant == des::kV2 || variant == des::kVis || enable_diffuse;
   const bool spec = variant != des::kDiffuse && enable_specular;
+  if (int rc = check_prepared(workspace, variant, diff, spec, true, ntc_forward)) return rc;
   const des::BatchLaunch bl = batch_launch(workspace, variant, diff, spec, true, ntc_forward);
   hipError_t e = des::launch_batch_forward(variant, diff, spec, bl, static_cast<hipStream_t>(stream));
   if (e != hipSuccess) return launch_error(e, "batched forward launch failed");
@@ -783,6 +836,7 @@ DES_API int des_batch_backward(int32_t variant, int enable_diffuse, int enable_s
   REQUIRE(workspace);
   const bool diff = variant == des::kDiffuse || variant == des::kV2 || variant == des::kVis || enable_diffuse;
   const bool spec = variant != des::kDiffuse && enable_specular;
+  if (int rc = check_prepared(workspace, variant, diff, spec, false, ntc_backward)) return rc;
   const des::BatchLaunch bl = batch_launch(workspace, variant, diff, spec, false, ntc_backward);
   hipError_t e = des::launch_batch_backward(variant, diff, spec, bl, static_cast<hipStream_t>(stream));
   if (e != hipSuccess) return launch_error(e, "batched backward launch failed");

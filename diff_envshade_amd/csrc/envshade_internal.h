@@ -74,11 +74,21 @@ struct BatchSet {
   int32_t colblk16;   // ceil(3 T / 16) (stage 2)
 };
 
+// Launch signature of a prepared batch: variant, branch flags and the waves per
+// tile of one direction.  The batched kernels compare the signature they were
+// launched with against the one recorded at prepare time and do nothing on a
+// mismatch (the host entry points reject the launch before that).
+__host__ __device__ constexpr int32_t batch_signature(int variant, bool diff, bool spec, int ntc) {
+  return int32_t(variant) | (diff ? 0x10 : 0) | (spec ? 0x20 : 0) | (int32_t(ntc) << 8);
+}
+
 struct BatchHeader {
   int32_t n_sets;
   int32_t variant;
   int32_t has_backward;  // every set carries upstream gradients and gradient outputs
   int32_t reserved;
+  int32_t sig_fwd;       // batch_signature(variant, diff, spec, ntc_forward) at prepare
+  int32_t sig_bwd;       // the same with ntc_backward
   int64_t total_tiles;
   int64_t total_r1;
   int64_t total_r2;
@@ -91,6 +101,7 @@ struct BatchLaunch {
   const BatchHeader* hdr;
   BatchHeader* hdr_mut;
   const BatchSet* sets;
+  int32_t sig;      // batch_signature of this launch
   int ntc;
   size_t lds;
   unsigned grid;

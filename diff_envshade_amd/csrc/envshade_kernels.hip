@@ -704,7 +704,9 @@ __device__ __forceinline__ int64_t next_item(unsigned long long* counter) {
 template <int V, bool DIFF, bool SPEC>
 __global__ __launch_bounds__(512) void batch_forward_kernel(const BatchHeader* __restrict__ hdr,
                                                              const BatchSet* __restrict__ sets,
-                                                             unsigned long long* counter) {
+                                                             unsigned long long* counter,
+                                                             int32_t sig) {
+  if (hdr->sig_fwd != sig) return;  // the table was prepared for another launch
   const int64_t total = hdr->total_tiles;
   const int n = hdr->n_sets;
   for (;;) {
@@ -718,10 +720,9 @@ __global__ __launch_bounds__(512) void batch_forward_kernel(const BatchHeader* _
 
 template <int V, bool DIFF, bool SPEC>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(3))) void batch_backward_kernel(
-    const BatchHeader* __restrict__ hdr,
-                                                             const BatchSet* __restrict__ sets,
-                                                             unsigned long long* counter) {
-  if (!hdr->has_backward) return;
+    const BatchHeader* __restrict__ hdr, const BatchSet* __restrict__ sets,
+    unsigned long long* counter, int32_t sig) {
+  if (!hdr->has_backward || hdr->sig_bwd != sig) return;
   const int64_t total = hdr->total_tiles;
   const int n = hdr->n_sets;
   for (;;) {
@@ -804,8 +805,8 @@ __global__ __launch_bounds__(256) void finish_envgrad_kernel(const double* __res
 }
 
 __global__ __launch_bounds__(256) void batch_reduce_partials_kernel(
-    const BatchHeader* __restrict__ hdr, const BatchSet* __restrict__ sets) {
-  if (!hdr->has_backward) return;
+    const BatchHeader* __restrict__ hdr, const BatchSet* __restrict__ sets, int32_t sig) {
+  if (!hdr->has_backward || hdr->sig_bwd != sig) return;
   const int64_t total = hdr->total_r1;
   const int n = hdr->n_sets;
   for (int64_t item = blockIdx.x; item < total; item += gridDim.x) {
@@ -820,8 +821,8 @@ __global__ __launch_bounds__(256) void batch_reduce_partials_kernel(
 }
 
 __global__ __launch_bounds__(256) void batch_finish_envgrad_kernel(
-    const BatchHeader* __restrict__ hdr, const BatchSet* __restrict__ sets) {
-  if (!hdr->has_backward) return;
+    const BatchHeader* __restrict__ hdr, const BatchSet* __restrict__ sets, int32_t sig) {
+  if (!hdr->has_backward || hdr->sig_bwd != sig) return;
   const int64_t total = hdr->total_r2;
   const int n = hdr->n_sets;
   for (int64_t item = blockIdx.x; item < total; item += gridDim.x) {
@@ -939,14 +940,14 @@ namespace {
 template <int V, bool DIFF, bool SPEC>
 hipError_t launch_bfwd(const BatchLaunch& bl, hipStream_t stream) {
   hipLaunchKernelGGL((batch_forward_kernel<V, DIFF, SPEC>), dim3(bl.grid), dim3(64 * bl.ntc),
-                     bl.lds, stream, bl.hdr, bl.sets, &bl.hdr_mut->fwd_counter);
+                     bl.lds, stream, bl.hdr, bl.sets, &bl.hdr_mut->fwd_counter, bl.sig);
   return hipGetLastError();
 }
 
 template <int V, bool DIFF, bool SPEC>
 hipError_t launch_bbwd(const BatchLaunch& bl, hipStream_t stream) {
   hipLaunchKernelGGL((batch_backward_kernel<V, DIFF, SPEC>), dim3(bl.grid), dim3(64 * bl.ntc),
-                     bl.lds, stream, bl.hdr, bl.sets, &bl.hdr_mut->bwd_counter);
+                     bl.lds, stream, bl.hdr, bl.sets, &bl.hdr_mut->bwd_counter, bl.sig);
   return hipGetLastError();
 }
 
@@ -996,11 +997,11 @@ hipError_t launch_batch_backward(int variant, bool diff, bool spec, const BatchL
   e = dispatch_batch<false>(variant, diff, spec, bl, stream);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(batch_reduce_partials_kernel, dim3(bl.reduce_grid), dim3(256), 0, stream,
-                     bl.hdr, bl.sets);
+                     bl.hdr, bl.sets, bl.sig);
   e = hipGetLastError();
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(batch_finish_envgrad_kernel, dim3(bl.reduce_grid), dim3(256), 0, stream,
-                     bl.hdr, bl.sets);
+                     bl.hdr, bl.sets, bl.sig);
   return hipGetLastError();
 }
 

@@ -281,7 +281,13 @@ DES_API int des_batch_plan(int32_t variant, const des_set* sets, int32_t n_sets,
 
 /* ntc_forward / ntc_backward: waves per 64-point tile (powers of two, <= 8),
  * usually from the autotuner; they must match the later launches.  with_backward:
- * the sets carry upstream gradients and gradient outputs. */
+ * the sets carry upstream gradients and gradient outputs.  The library records
+ * (variant, flags, ntc_forward, ntc_backward, with_backward) per workspace and
+ * the table carries them too: des_batch_forward / des_batch_backward /
+ * des_batch_sort on a workspace that was not prepared, or with another variant,
+ * other flags or another ntc, return DES_ERR_INVALID_ARGUMENT before any
+ * launch, and a replayed graph whose table was re-prepared for another launch
+ * does nothing on the device. */
 DES_API int des_batch_prepare(int32_t variant, int enable_diffuse, int enable_specular,
                               const des_set* sets, int32_t n_sets, int32_t ntc_forward,
                               int32_t ntc_backward, int with_backward, void* staging,
@@ -291,7 +297,7 @@ DES_API int des_batch_prepare(int32_t variant, int enable_diffuse, int enable_sp
 DES_API int des_batch_forward(int32_t variant, int enable_diffuse, int enable_specular,
                               int32_t ntc_forward, void* workspace, void* stream);
 
-/* Does nothing on the device when the batch was prepared without with_backward. */
+/* DES_ERR_INVALID_ARGUMENT when the batch was prepared without with_backward. */
 DES_API int des_batch_backward(int32_t variant, int enable_diffuse, int enable_specular,
                                int32_t ntc_backward, void* workspace, void* stream);
 

@@ -309,3 +309,16 @@ def test_call_drops_the_hand_over_when_the_call_cannot_be_made(monkeypatch):
     with pytest.raises(ctypes.ArgumentError):
         _lib.call("des_bruteforce_specular_shader_backward2", "x", order=(DUMMY, 10))
     assert hooks == [(DUMMY, 10), (None, 0)]
+
+
+def test_batch_launches_reject_an_unprepared_workspace():
+    # des_batch_forward / _backward / _sort on a workspace that no
+    # des_batch_prepare wrote return before any launch
+    from diff_envshade_amd.batch import _bind
+    L = _bind(_lib.load())
+    assert L.des_batch_forward(4, 1, 1, 8, DUMMY, None) == -1
+    assert b"not prepared" in L.des_last_error()
+    assert L.des_batch_backward(4, 1, 1, 8, DUMMY, None) == -1
+    assert b"not prepared" in L.des_last_error()
+    assert L.des_batch_sort(DUMMY, None) == -1
+    assert b"not prepared" in L.des_last_error()

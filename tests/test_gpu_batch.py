@@ -121,7 +121,8 @@ def test_forward_only_batch():
     b = ShadeBatch("vis")
     outs = b.prepare(devs, with_backward=False)
     b.forward()
-    b.backward()  # no-op on a forward-only batch
+    with pytest.raises(RuntimeError, match="without backward"):
+        b.backward()  # rejected before any launch
     torch.cuda.synchronize()
     for d, o in zip(nps, outs):
         sh, ds = oracle_run("vis", d)[0]
@@ -248,3 +249,34 @@ def test_envgrad_reduce_of_single_set_matches_batch_at_many_groups(order):
     finally:
         _lib.set_tiling_override(0, 0)
         _lib.set_point_order(0)
+
+
+def test_launch_must_match_what_was_prepared():
+    # a launch with another variant, other flags or another ntc than the
+    # preparation recorded is rejected before it runs (it would read pointers
+    # the preparation did not check, or deal groups to missing waves)
+    from diff_envshade_amd import _lib
+    from diff_envshade_amd.batch import ShadeBatch, _bind
+    L = _bind(_lib.load())
+    nps, devs = _sets([(3000, 16, 32)], seed=61)
+    b = ShadeBatch("vis", enable_specular=False, ntc_forward=4, ntc_backward=2)
+    b.prepare(devs, with_backward=True)
+    ws = b.workspace.data_ptr()
+    s = torch.cuda.current_stream().cuda_stream
+    assert L.des_batch_forward(4, 1, 1, 4, ws, s) == -1      # specular on, prepared off
+    assert b"specular" in L.des_last_error()
+    assert L.des_batch_forward(4, 1, 0, 2, ws, s) == -1      # another ntc_forward
+    assert b"ntc_forward" in L.des_last_error()
+    assert L.des_batch_backward(4, 1, 0, 4, ws, s) == -1     # another ntc_backward
+    assert b"ntc_backward" in L.des_last_error()
+    assert L.des_batch_backward(3, 1, 0, 2, ws, s) == -1     # another variant
+    assert L.des_batch_forward(4, 1, 0, 4, ws, s) == 0
+    assert L.des_batch_backward(4, 1, 0, 2, ws, s) == 0
+    torch.cuda.synchronize()
+    d = nps[0]
+    ref = oracle_run("vis", d, spec=False)
+    got = ([b.outputs[0]["shading"].cpu().numpy(), b.outputs[0]["diffuse_shading"].cpu().numpy()],
+           [b.outputs[0][k].cpu().numpy() for k in ("grad_normal", "grad_view_dir", "grad_albedo",
+                                                    "grad_specular_albedo", "grad_roughness",
+                                                    "grad_envmap")])
+    compare("vis", got, ref)
