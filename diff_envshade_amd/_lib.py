@@ -44,6 +44,8 @@ _SIGS = {
     "des_bruteforce_specularvisibility_shader_backward2":
         [_i64, _i32, _i32, _i32, _i32, _i32, _i32] + [_vp] * 13 + [_int] + [_vp] * 6
         + [_vp, _sz, _vp],
+    "des_bruteforce_specularvisibility_shader_backward2_noalloc":
+        [_i64, _i32, _i32, _i32, _i32, _i32, _i32] + [_vp] * 13 + [_int] + [_vp] * 6 + [_vp],
     "des_get_nearest_mesh_points_forward":
         [_i32, _i64, _i64, _i32, _vp, _vp, _i64] + [_vp] * 6 + [_vp],
     "des_get_nearest_mesh_points_backward":
@@ -109,6 +111,8 @@ def load() -> ctypes.CDLL:
         L.des_set_backward_timing_events.restype = _int
         L.des_reuse_point_order_once.argtypes = [_vp, _i64]
         L.des_reuse_point_order_once.restype = _int
+        L.des_static_scratch_bytes.argtypes = []
+        L.des_static_scratch_bytes.restype = _sz
         _lib = L
     return _lib
 

@@ -381,6 +381,46 @@ des::BatchLaunch batch_launch(void* ws, int variant, bool diff, bool spec, bool 
   return bl;
 }
 
+// ---- scratch of the entry point that allocates nothing -------------------------
+//
+// des_bruteforce_specularvisibility_shader_backward2_noalloc keeps its
+// workspace (per-tile env-map-gradient partials, FP64 group sums, point order)
+// in this array instead of a caller's buffer.  Each GPU holds its own copy
+// from the library's load to the end of the process; in host code the name
+// is only a stand-in, so the device address of each GPU's copy comes from
+// hipGetSymbolAddress, looked up once per device with that device current and
+// used wherever the host hands the scratch to the device.  Every call on one
+// GPU uses the same array: calls that overlap on one GPU (other streams,
+// other threads) race.  Every byte a call reads is written earlier in the same
+// call, so nothing needs clearing between calls.
+constexpr size_t kStaticScratchBytes = size_t(32) << 20;
+static __device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of the current device's scratch into *out (DES_OK), or an error.
+int device_scratch(void** out) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return launch_error(e, "hipGetDevice failed");
+  if (dev < 0 || dev >= kMaxDevices)
+    return fail(DES_ERR_INVALID_ARGUMENT, "device %d is beyond the %d the scratch table covers",
+                dev, kMaxDevices);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[dev] == nullptr) {
+    void* ptr = nullptr;
+    e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return launch_error(e, "hipGetSymbolAddress of the static scratch failed");
+    if (ptr == nullptr)
+      return fail(DES_ERR_LAUNCH, "hipGetSymbolAddress gave no address for the static scratch");
+    g_scratch_ptr[dev] = ptr;
+  }
+  *out = g_scratch_ptr[dev];
+  return DES_OK;
+}
+
 // What des_batch_prepare recorded for a workspace, so that a launch with
 // another variant, other branch flags or another launch shape is rejected
 // before it runs (its kernels would read pointers the preparation did not
@@ -686,6 +726,35 @@ DES_API int des_bruteforce_specularvisibility_shader_backward2(
                            true, enable_specular != 0, grad_normal, grad_view_dir, grad_albedo,
                            grad_specular_albedo, grad_roughness, grad_envmap, workspace,
                            workspace_bytes, stream, &pr);
+}
+
+DES_API size_t des_static_scratch_bytes(void) { return kStaticScratchBytes; }
+
+DES_API int des_bruteforce_specularvisibility_shader_backward2_noalloc(
+    int64_t P, int32_t C, int32_t H, int32_t W, int32_t N_face, int32_t N_vertex, int32_t N_slot,
+    const float* grad_shading, const float* grad_diffuse_shading, const float* normal,
+    const float* view_dir, const float* albedo, const float* specular_albedo,
+    const float* roughness, const float* envmap, const float* light2obj_rotmat,
+    const int32_t* face_vertex_list, const int64_t* nearest_triangle_id,
+    const float* barycentric_coord, const int32_t* visibility, int enable_specular,
+    float* grad_normal, float* grad_view_dir, float* grad_albedo, float* grad_specular_albedo,
+    float* grad_roughness, float* grad_envmap, void* stream) {
+  if (int rc = check_common(P, C, H, W)) return rc;
+  const size_t need = des_backward_workspace_size(P, H, W);
+  if (need > kStaticScratchBytes)
+    return fail(DES_ERR_INVALID_ARGUMENT,
+                "P = %lld points with a %d x %d env map need %zu bytes of scratch; the static "
+                "scratch of this entry point holds %zu (use the workspace entry point)",
+                (long long)P, H, W, need, kStaticScratchBytes);
+  void* ws = nullptr;
+  if (P > 0)
+    if (int rc = device_scratch(&ws)) return rc;
+  return des_bruteforce_specularvisibility_shader_backward2(
+      P, C, H, W, N_face, N_vertex, N_slot, grad_shading, grad_diffuse_shading, normal, view_dir,
+      albedo, specular_albedo, roughness, envmap, light2obj_rotmat, face_vertex_list,
+      nearest_triangle_id, barycentric_coord, visibility, enable_specular, grad_normal,
+      grad_view_dir, grad_albedo, grad_specular_albedo, grad_roughness, grad_envmap, ws,
+      ws ? kStaticScratchBytes : 0, stream);
 }
 
 DES_API int des_batch_plan(int32_t variant, const des_set* sets, int32_t n_sets,

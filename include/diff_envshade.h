@@ -217,6 +217,32 @@ DES_API int des_bruteforce_specularvisibility_shader_backward2(
  * choice).  Process-wide; results do not depend on it beyond fp32 rounding. */
 DES_API int des_set_tiling_override(int32_t ntc_forward, int32_t ntc_backward);
 
+/* ---- backward without allocation (no counterpart in the reference) ---------
+ * For callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc): the
+ * visibility backward (des_bruteforce_specularvisibility_shader_backward2,
+ * same arguments, same results bit for bit) with its workspace in a static
+ * per-device scratch array of des_static_scratch_bytes() bytes (32 MiB) that
+ * the library holds from its load to the end of the process.  Problems whose
+ * des_backward_workspace_size exceeds it are rejected with
+ * DES_ERR_INVALID_ARGUMENT: up to 279 424 points with a 16 x 32 map, up to
+ * 75 456 points with a 32 x 64 map (70 000 points at 16 x 32 use 8.4 MB).
+ * Nothing is allocated: the
+ * first call on a device only looks up that device's scratch address.  All
+ * calls on one GPU share its scratch, so calls that may overlap on one GPU
+ * (other streams, other threads) race and must be serialised by the caller. */
+DES_API size_t des_static_scratch_bytes(void);
+
+DES_API int des_bruteforce_specularvisibility_shader_backward2_noalloc(
+    int64_t P, int32_t C, int32_t H, int32_t W, int32_t N_face, int32_t N_vertex, int32_t N_slot,
+    const float* grad_shading, const float* grad_diffuse_shading, const float* normal,
+    const float* view_dir, const float* albedo, const float* specular_albedo,
+    const float* roughness, const float* envmap, const float* light2obj_rotmat,
+    const int32_t* face_vertex_list, const int64_t* nearest_triangle_id,
+    const float* barycentric_coord, const int32_t* visibility, int enable_specular,
+    float* grad_normal, float* grad_view_dir, float* grad_albedo, float* grad_specular_albedo,
+    float* grad_roughness, float* grad_envmap, void* stream);
+
 /* ---- batched launches (no counterpart in the reference) --------------------
  * Many point sets, each with its own env map and rotation, shaded by one
  * persistent launch per direction.  All sets of a batch use one variant and
