@@ -76,7 +76,10 @@ class ShadeBatch:
 
     def __init__(self, variant: str = "vis", enable_diffuse: bool = True,
                  enable_specular: bool = True, ntc_forward: int = 8, ntc_backward: int = 8,
-                 device=None):
+                 device=None, workspace=None):
+        """workspace: a device uint8 tensor to prepare batches into (fixed size;
+        a batch that needs more is refused); by default the batch allocates one
+        and grows it as needed."""
         self.variant = variant
         self.vid = VARIANTS[variant]
         self.enable_diffuse = bool(enable_diffuse)
@@ -86,7 +89,8 @@ class ShadeBatch:
         self.device = torch.device(device) if device is not None else torch.device(
             "cuda", torch.cuda.current_device())
         self.L = _bind(_lib.load())
-        self.workspace = None
+        self.fixed_workspace = workspace is not None
+        self.workspace = workspace
         self.staging = None
         self._staging_ready = None
         self.graph = None
@@ -145,8 +149,10 @@ class ShadeBatch:
         st = ctypes.c_size_t()
         _lib.check(self.L.des_batch_plan(self.vid, arr, n, ctypes.byref(ws), ctypes.byref(st)))
         if self.workspace is None or self.workspace.numel() < ws.value:
-            if self.graph is not None:
-                raise RuntimeError("batch exceeds the workspace the captured graph was built on")
+            if self.graph is not None or self.fixed_workspace:
+                have = 0 if self.workspace is None else self.workspace.numel()
+                raise RuntimeError(f"batch needs a workspace of {ws.value} bytes, the fixed "
+                                   f"one holds {have}")
             self.workspace = torch.empty(max(ws.value, 256), dtype=torch.uint8,
                                          device=self.device)
         # the previous upload must have consumed the staging bytes before they are
@@ -187,15 +193,23 @@ class ShadeBatch:
         s = torch.cuda.current_stream(self.device).cuda_stream
         _lib.check(self.L.des_batch_sort(self.workspace.data_ptr(), s))
 
-    def capture(self, with_backward: bool = True):
+    def release(self):
+        """Drop the references to the last batch's inputs and outputs (call once
+        its launches have completed; the workspace stays)."""
+        self.outputs = []
+        self._sets = None
+
+    def capture(self, with_backward: bool = True, sort: bool = True):
         """Capture sort + forward (+ backward) into a HIP graph; replay() reruns
-        it for whatever batch was last prepared into this workspace."""
+        it for whatever batch was last prepared into this workspace.  sort=False
+        leaves the point order out of the graph (prepare() computes it)."""
         if self.workspace is None:
             raise RuntimeError("prepare() a batch before capturing")
         torch.cuda.synchronize(self.device)
         g = torch.cuda.CUDAGraph()
         with torch.cuda.graph(g):
-            self.sort()
+            if sort:
+                self.sort()
             self.forward()
             if with_backward:
                 self.backward()

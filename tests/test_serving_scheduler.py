@@ -127,3 +127,52 @@ def test_request_above_budget_is_rejected():
     r.bytes = request_bytes(r.points, r.texels, r.backward)
     with pytest.raises(RuntimeError, match="above the whole HBM budget"):
         _run([r], budget=1 << 20)
+
+
+def test_burst_arrivals_come_together_and_stay_ordered():
+    wl = PoissonWorkload(n_requests=200, rate=1000.0, seed=3, burst=(50, 40))
+    a = wl.arrivals
+    assert (a[50:90] == a[50]).all()
+    assert (a[1:] >= a[:-1]).all()
+    assert a[90] > a[89]
+
+
+def test_max_overlap():
+    from diff_envshade_amd.serving import max_overlap
+    assert max_overlap([]) == 0
+    assert max_overlap([(0, 1), (1, 2)]) == 1  # touching intervals do not overlap
+    assert max_overlap([(0, 3), (1, 2), (1.5, 4)]) == 3
+
+
+class WsSimExecutor(SimExecutor):
+    """A simulated executor with preallocated batch workspaces."""
+
+    def __init__(self, clock, slots, ws_bytes):
+        super().__init__(clock, slots=slots)
+        self.fixed_bytes = slots * ws_bytes
+        self.batch_workspace_bytes = ws_bytes
+        self.batch_ws = []
+
+    def launch(self, kind, requests):
+        self.batch_ws.append(sum(r.ws_bytes for r in requests))
+        return super().launch(kind, requests)
+
+
+def test_preallocated_workspaces_are_charged_up_front_and_bound_each_batch():
+    wl = PoissonWorkload(n_requests=300, rate=1e6, seed=4)
+    reqs = wl.requests()
+    assert all(0 < r.ws_bytes < r.bytes for r in reqs)
+    clk = VirtualClock()
+    ws = 200 << 20
+    ex = WsSimExecutor(clk, slots=3, ws_bytes=ws)
+    budget = 3 * ws + (300 << 20)
+    sch = Scheduler(ex, budget, clock=clk)
+    assert sch.budget == 300 << 20
+    sch.run(reqs, on_idle=clk.tick)
+    assert all(r.done_s is not None for r in reqs)
+    assert max(ex.batch_ws) <= ws
+    assert sch.peak_in_use <= 300 << 20
+    # a request is charged its outputs and gradients, not its workspace share
+    assert sch.charge(reqs[0]) == reqs[0].bytes - reqs[0].ws_bytes
+    with pytest.raises(RuntimeError, match="more than the HBM budget"):
+        Scheduler(WsSimExecutor(clk, slots=3, ws_bytes=ws), 3 * ws, clock=clk)
