@@ -99,8 +99,36 @@ des::Tiling tiling_for(int ntc, int T, bool backward) {
   tl.lds_fwd = std::max(staged, size_t(ntc) * des::kForwardAcc * 64 * sizeof(float));
   tl.lds_bwd = std::max(staged, size_t(ntc) * des::kBackwardAcc * 64 * sizeof(float));
   tl.reduce_rows = des::kReduceRows;
+  tl.split0 = INT64_MAX;  // set per problem by choose_tiling (single-set launches only)
+  tl.slices = 1;
+  tl.slice_texels = T;
   (void)backward;
   return tl;
+}
+
+// Backward tail slices.  A backward launch keeps about 1024 tiles resident on
+// MI355X (256 CUs x four 4-wave tiles); the tiles beyond a multiple of that
+// would otherwise run as a fifth tile on some CUs while the others idle
+// (measured at 70 000 points: 70 CUs did 5 tiles, the rest 4).  When that
+// remainder is at most half the resident count, its tiles are shaded as four
+// texel slices each, after the whole tiles, so every CU gets about the same
+// work.  The rule depends on (P, T) only, so results do not depend on the
+// device.
+constexpr int64_t kResidentTiles = 1024;
+constexpr int kTailSlices = 4;
+
+struct TailSplit {
+  int64_t split0;
+  int slices, slice_texels;
+};
+
+TailSplit tail_split(int64_t P, int T) {
+  const int64_t tiles = n_tiles(P);
+  const int nslot = (T + 31) / 32;
+  const int64_t rem = tiles % kResidentTiles;
+  if (tiles > kResidentTiles && rem > 0 && 2 * rem <= kResidentTiles && nslot >= kTailSlices)
+    return TailSplit{tiles - rem, kTailSlices, ((nslot + kTailSlices - 1) / kTailSlices) * 32};
+  return TailSplit{tiles, 1, T};
 }
 
 // Launch shape as a function of (P, T) only, so results never depend on the
@@ -123,7 +151,15 @@ des::Tiling choose_tiling(int64_t P, int T, bool backward) {
     while (ntc * 2 <= base && ntc * 2 <= nslot) ntc *= 2;
     while (ntc * 2 <= cap && ntc * 2 <= nslot && tiles * ntc < kMinWaves) ntc *= 2;
   }
-  return tiling_for(ntc, T, backward);
+  des::Tiling tl = tiling_for(ntc, T, backward);
+  tl.split0 = tiles;
+  if (backward && ov <= 0 && ntc == 4) {
+    const TailSplit ts = tail_split(P, T);
+    tl.split0 = ts.split0;
+    tl.slices = ts.slices;
+    tl.slice_texels = ts.slice_texels;
+  }
+  return tl;
 }
 
 size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
@@ -137,20 +173,27 @@ bool use_order(int64_t P, int T, bool backward) {
 }
 
 struct WsLayout {
-  size_t part_off, part2_off, order_off, total;
+  size_t part_off, part2_off, spart_off, order_off, total;
   bool ordered;
 };
 
-// Backward workspace: per-tile fp32 partials, fp64 group sums, and (when the
-// points are reordered) the point-order workspace.
+// Backward workspace: per-tile fp32 partials, fp64 group sums, the per-point
+// sums of the tail slices (single-set launches), and (when the points are
+// reordered) the point-order workspace.
 WsLayout ws_layout(int64_t P, int T, bool allow_order) {
-  WsLayout w{0, 0, 0, 0, false};
+  WsLayout w{0, 0, 0, 0, 0, false};
   if (P <= 0) return w;
   const int64_t tiles = n_tiles(P);
   const int64_t groups = (tiles + des::kReduceRows - 1) / des::kReduceRows;
   w.part_off = 0;
   w.part2_off = align256(size_t(tiles) * 3 * T * sizeof(float));
   w.total = w.part2_off + align256(size_t(groups) * 3 * T * sizeof(double));
+  if (allow_order) {
+    const TailSplit ts = tail_split(P, T);
+    w.spart_off = w.total;
+    w.total += align256(size_t(tiles - ts.split0) * ts.slices * des::kBackwardAcc * 64 *
+                        sizeof(float));
+  }
   if (allow_order && use_order(P, T, true)) {
     w.ordered = true;
     w.order_off = w.total;
@@ -233,6 +276,7 @@ int run_backward(int variant, bool diff, bool spec, des::Problem& pr, void* ws, 
     return fail(DES_ERR_INVALID_ARGUMENT, "workspace must be 256-byte aligned");
   pr.part = reinterpret_cast<float*>(static_cast<char*>(ws) + lay.part_off);
   pr.part2 = reinterpret_cast<double*>(static_cast<char*>(ws) + lay.part2_off);
+  pr.spart = reinterpret_cast<float*>(static_cast<char*>(ws) + lay.spart_off);
   if (lay.ordered && reuse != nullptr) {
     pr.order = reinterpret_cast<const int32_t*>(static_cast<const char*>(reuse) +
                                                 des::order_layout(pr.P).order_off);
@@ -255,7 +299,7 @@ int run_backward(int variant, bool diff, bool spec, des::Problem& pr, void* ws, 
     hipError_t er = hipEventRecord(ev1, stream);
     if (er != hipSuccess) return launch_error(er, "timing event record failed");
   }
-  e = des::launch_envgrad_reduce(pr, tl, stream);
+  e = des::launch_envgrad_reduce(variant, diff, spec, pr, tl, stream);
   if (e != hipSuccess) return launch_error(e, "env-map gradient reduce failed");
   return ok();
 }

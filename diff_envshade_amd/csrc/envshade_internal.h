@@ -44,6 +44,7 @@ struct Problem {
   const int32_t* order;   // P: point of each tile lane (tile*64 + lane), or null = identity
   float* part;            // [ceil(P/64)][3][T] per-tile env-map partials
   double* part2;          // [ngroups][3][T] stage-1 FP64 sums
+  float* spart;           // backward tail slices: [slice item][kBackwardAcc][64] per-point sums
 };
 
 // Launch shape: one 64-point tile per workgroup, NTC waves over texel chunks.
@@ -55,6 +56,13 @@ struct Tiling {
   size_t lds_fwd;   // dynamic LDS bytes, forward
   size_t lds_bwd;   // dynamic LDS bytes, backward
   int reduce_rows;  // partial rows summed per stage-1 group
+  // Backward tail slices: tiles [split0, tiles) are shaded as `slices`
+  // workgroups each, slice s covering texels [s * slice_texels, ...), after the
+  // whole tiles; their per-point sums are combined by the stage-1 reduce.
+  // split0 = tiles and slices = 1 when nothing is sliced.
+  int64_t split0;
+  int slices;
+  int slice_texels;
 };
 
 struct SortJobs;  // the point-order jobs of a batch (below)
@@ -124,7 +132,10 @@ hipError_t launch_forward(int variant, bool diff, bool spec, const Problem& pr, 
                           hipStream_t stream);
 hipError_t launch_backward(int variant, bool diff, bool spec, const Problem& pr, const Tiling& tl,
                            hipStream_t stream);
-hipError_t launch_envgrad_reduce(const Problem& pr, const Tiling& tl, hipStream_t stream);
+// Stage 1 (and the combine of the backward's tail slices) and stage 2 of the
+// env-map-gradient reduction.
+hipError_t launch_envgrad_reduce(int variant, bool diff, bool spec, const Problem& pr,
+                                 const Tiling& tl, hipStream_t stream);
 hipError_t launch_fill_zero(float* x, int64_t n, hipStream_t stream);
 
 // ---- normal-coherent point order (point_order.hip) ----

@@ -385,7 +385,7 @@ __device__ __forceinline__ void forward_tile(const Problem& pr, const Tiling& tl
   for (int s0 = 0; s0 < pr.T; s0 += tl.st) {
     const int sn = min(tl.st, pr.T - s0);
     __syncthreads();
-    build_table(pr, s0, sn, tl.st, tab);
+    build_table(pr, s0, sn, min(tl.st, (sn + 31) & ~31), tab);
     stage_vis(pr, st, s0, sn, single, svis, wv, tl.ntc, lane);
     __syncthreads();
     if (single)
@@ -544,6 +544,43 @@ __device__ __forceinline__ void backward_chunk(const PointState<V, SPEC>& st, co
   });
 }
 
+// Per-point gradients of point p from its reduced sums acc (kBackwardAcc
+// entries, see BwdSums) and its upstream shading gradient gs.
+template <int V, bool DIFF, bool SPEC>
+__device__ __forceinline__ void write_backward_outputs(const Problem& pr,
+                                                       const PointGeom& st, int64_t p,
+                                                       const float* gs, const float* acc) {
+  // gn = sum (coef_n l + rs gdnh hb) + v [n.v>0](1-k) Sb
+  // gv = sum (rs gch l - rs^2 gch (l.h) hb + rs^3 gdnh (hb x (n x hb)))
+  //      + n [n.v>0](1-k) Sb
+  const float Sb = -4.f * acc[12];
+  const float nvpos = (st.nv > 0.f) ? 1.f : 0.f;
+  const float g1v = SPEC ? nvpos * st.omk * Sb : 0.f;
+  pr.gn[3 * p + 0] = fmaf(st.vx, g1v, acc[6]);
+  pr.gn[3 * p + 1] = fmaf(st.vy, g1v, acc[7]);
+  pr.gn[3 * p + 2] = fmaf(st.vz, g1v, acc[8]);
+  if (V != kDiffuse) {
+    // disabled branches are written as zeros (bs:2889-2905, bs:3205-3214)
+    pr.gv[3 * p + 0] = SPEC ? fmaf(st.nx, g1v, acc[9]) : 0.f;
+    pr.gv[3 * p + 1] = SPEC ? fmaf(st.ny, g1v, acc[10]) : 0.f;
+    pr.gv[3 * p + 2] = SPEC ? fmaf(st.nz, g1v, acc[11]) : 0.f;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) pr.gs[3 * p + c] = SPEC ? gs[c] * acc[3 + c] : 0.f;
+    float gr = 0.f;
+    if (SPEC) {
+      const float r = pr.rough[p];
+      const float gkt = fmaf(1.f - fmaxf(st.nv, 0.f), Sb, acc[15]);
+      // grad of alpha^2: sum gD it2 + sum (n.h)^2 g_t = ga2a / pi - 2 ga2b
+      const float ga2 = fmaf(acc[13], kInvPi, -2.f * acc[14]);
+      // bs:2501-2502
+      gr = fmaf(ga2 * 4.f * r * r, r, gkt * (1.f + r) * 0.25f);
+    }
+    pr.gr[p] = gr;
+  }
+#pragma unroll
+  for (int c = 0; c < 3; ++c) pr.ga[3 * p + c] = DIFF ? gs[c] * acc[c] * kInvPi : 0.f;
+}
+
 // ---- backward ----------------------------------------------------------------
 //
 // Per pair (with x = vis max(l.n,0), y = vis F G D / 4, E'_c = E_c dOmega,
@@ -556,9 +593,13 @@ __device__ __forceinline__ void backward_chunk(const PointState<V, SPEC>& st, co
 //   terms of the NDF and normalize VJPs are accumulated along l + v and along
 //   hb x (n x hb); the G1 parts along n and v are per-point constants times a
 //   scalar sum (Sb), applied once at the end.
+// Texels [t_begin, t_end) of one 64-point tile (t_begin a multiple of 32);
+// slice_item >= 0 marks a tail slice, whose per-point sums go to
+// pr.spart[slice_item] instead of the outputs.
 template <int V, bool DIFF, bool SPEC>
 __device__ __forceinline__ void backward_tile(const Problem& pr, const Tiling& tl, int64_t tile,
-                                              const int32_t* order) {
+                                              const int32_t* order, int t_begin, int t_end,
+                                              int64_t slice_item) {
   extern __shared__ float4 smem[];
   constexpr bool DOUT = (V == kV2 || V == kVis);
   const int lane = threadIdx.x & 63;
@@ -601,10 +642,10 @@ __device__ __forceinline__ void backward_tile(const Problem& pr, const Tiling& t
   const int row_texel = (row == 1) ? 2 : ((row == 2) ? 1 : row);
   const int ch_out = lane & 15;
 
-  for (int s0 = 0; s0 < pr.T; s0 += tl.st) {
-    const int sn = min(tl.st, pr.T - s0);
+  for (int s0 = t_begin; s0 < t_end; s0 += tl.st) {
+    const int sn = min(tl.st, t_end - s0);
     __syncthreads();
-    build_table(pr, s0, sn, tl.st, tab);
+    build_table(pr, s0, sn, min(tl.st, (sn + 31) & ~31), tab);
     stage_vis(pr, st, s0, sn, single, svis, wv, tl.ntc, lane);
     __syncthreads();
     backward_chunk<V, DIFF, SPEC>(st, tab, svis, cone, s0, sn, wv, tl.ntc, lane, single, csum, A,
@@ -629,36 +670,15 @@ __device__ __forceinline__ void backward_tile(const Problem& pr, const Tiling& t
       acc[j] = sum;
     }
   }
-  if (!valid) return;
-  // gn = sum (coef_n l + rs gdnh hb) + v [n.v>0](1-k) Sb
-  // gv = sum (rs gch l - rs^2 gch (l.h) hb + rs^3 gdnh (hb x (n x hb)))
-  //      + n [n.v>0](1-k) Sb
-  const float Sb = -4.f * acc[12];
-  const float nvpos = (st.nv > 0.f) ? 1.f : 0.f;
-  const float g1v = SPEC ? nvpos * st.omk * Sb : 0.f;
-  pr.gn[3 * p + 0] = fmaf(st.vx, g1v, acc[6]);
-  pr.gn[3 * p + 1] = fmaf(st.vy, g1v, acc[7]);
-  pr.gn[3 * p + 2] = fmaf(st.vz, g1v, acc[8]);
-  if (V != kDiffuse) {
-    // disabled branches are written as zeros (bs:2889-2905, bs:3205-3214)
-    pr.gv[3 * p + 0] = SPEC ? fmaf(st.nx, g1v, acc[9]) : 0.f;
-    pr.gv[3 * p + 1] = SPEC ? fmaf(st.ny, g1v, acc[10]) : 0.f;
-    pr.gv[3 * p + 2] = SPEC ? fmaf(st.nz, g1v, acc[11]) : 0.f;
+  if (slice_item >= 0) {
+    // a tail slice: its per-point sums, combined after the launch
+    float* out = pr.spart + (int64_t)slice_item * NACC * 64;
 #pragma unroll
-    for (int c = 0; c < 3; ++c) pr.gs[3 * p + c] = SPEC ? gs[c] * acc[3 + c] : 0.f;
-    float gr = 0.f;
-    if (SPEC) {
-      const float r = pr.rough[p];
-      const float gkt = fmaf(1.f - fmaxf(st.nv, 0.f), Sb, acc[15]);
-      // grad of alpha^2: sum gD it2 + sum (n.h)^2 g_t = ga2a / pi - 2 ga2b
-      const float ga2 = fmaf(acc[13], kInvPi, -2.f * acc[14]);
-      // bs:2501-2502
-      gr = fmaf(ga2 * 4.f * r * r, r, gkt * (1.f + r) * 0.25f);
-    }
-    pr.gr[p] = gr;
+    for (int j = 0; j < NACC; ++j) out[j * 64 + lane] = acc[j];
+    return;
   }
-#pragma unroll
-  for (int c = 0; c < 3; ++c) pr.ga[3 * p + c] = DIFF ? gs[c] * acc[c] * kInvPi : 0.f;
+  if (!valid) return;
+  write_backward_outputs<V, DIFF, SPEC>(pr, st, p, gs, acc);
 }
 
 // ---- single-problem launches: one workgroup per tile ---------------------------
@@ -672,7 +692,16 @@ __global__ __launch_bounds__(1024) void shade_forward_kernel(Problem pr, Tiling 
 template <int V, bool DIFF, bool SPEC>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void shade_backward_kernel(
     Problem pr, Tiling tl) {
-  backward_tile<V, DIFF, SPEC>(pr, tl, blockIdx.x, pr.order);
+  const int64_t b = blockIdx.x;
+  if (b < tl.split0) {
+    backward_tile<V, DIFF, SPEC>(pr, tl, b, pr.order, 0, pr.T, -1);
+  } else {
+    // tail slices, dispatched after the whole tiles
+    const int64_t j = b - tl.split0;
+    const int64_t tile = tl.split0 + j / tl.slices;
+    const int t0 = (int)(j % tl.slices) * tl.slice_texels;
+    backward_tile<V, DIFF, SPEC>(pr, tl, tile, pr.order, t0, min(pr.T, t0 + tl.slice_texels), j);
+  }
 }
 
 // ---- batched launches: persistent workgroups pull tiles of many problems ------
@@ -730,7 +759,8 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(3))) void b
     if (g >= total) break;
     const int s = __builtin_amdgcn_readfirstlane(
         find_set(n, g, [&](int i) { return sets[i].tile0; }));
-    backward_tile<V, DIFF, SPEC>(sets[s].pr, sets[s].tl_bwd, g - sets[s].tile0, sets[s].order_bwd);
+    backward_tile<V, DIFF, SPEC>(sets[s].pr, sets[s].tl_bwd, g - sets[s].tile0, sets[s].order_bwd,
+                                 0, sets[s].pr.T, -1);
   }
 }
 
@@ -792,10 +822,41 @@ __device__ __forceinline__ void finish_item(const double* __restrict__ part2,
   __syncthreads();
 }
 
-__global__ __launch_bounds__(256) void reduce_partials_kernel(const float* __restrict__ part,
-                                                              double* __restrict__ part2,
-                                                              int64_t nrows, int ncol, int R) {
-  reduce_rows_item(part, part2, nrows, ncol, R, blockIdx.y, blockIdx.x);
+// Stage 1, and in rows blockIdx.y >= ngroups (blockIdx.x == 0) the combine of
+// the backward's tail slices: per point the slices' sums in slice order, then
+// the point's gradients.
+template <int V, bool DIFF, bool SPEC>
+__global__ __launch_bounds__(256) void reduce_partials_kernel(Problem pr, Tiling tl,
+                                                              int64_t nrows, int ncol, int R,
+                                                              int ngroups) {
+  if ((int)blockIdx.y < ngroups) {
+    reduce_rows_item(pr.part, pr.part2, nrows, ncol, R, blockIdx.y, blockIdx.x);
+    return;
+  }
+  if (blockIdx.x != 0) return;
+  const int64_t idx = (int64_t)(blockIdx.y - ngroups) * 256 + threadIdx.x;
+  const int64_t ntail = nrows - tl.split0;  // >= 0 (checked at launch)
+  if (idx >= ntail * 64) return;
+  const int64_t k = idx >> 6;
+  const int lane = (int)(idx & 63);
+  const int64_t q = (tl.split0 + k) * 64 + lane;
+  if (q >= pr.P) return;
+  const int64_t p = pr.order != nullptr ? (int64_t)pr.order[q] : q;
+  constexpr int NACC = kBackwardAcc;
+  float acc[NACC];
+#pragma unroll
+  for (int j = 0; j < NACC; ++j) acc[j] = 0.f;
+  for (int sl = 0; sl < tl.slices; ++sl) {
+    const float* in = pr.spart + (k * tl.slices + sl) * NACC * 64;
+#pragma unroll
+    for (int j = 0; j < NACC; ++j) acc[j] += in[j * 64 + lane];
+  }
+  PointState<V, SPEC> st;
+  load_point<V, SPEC>(pr, p, true, st);
+  float gs[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) gs[c] = pr.g_sh[3 * p + c];
+  write_backward_outputs<V, DIFF, SPEC>(pr, st, p, gs, acc);
 }
 
 __global__ __launch_bounds__(256) void finish_envgrad_kernel(const double* __restrict__ part2,
@@ -853,7 +914,8 @@ hipError_t launch_fwd(const Problem& pr, const Tiling& tl, hipStream_t stream) {
 
 template <int V, bool DIFF, bool SPEC>
 hipError_t launch_bwd(const Problem& pr, const Tiling& tl, hipStream_t stream) {
-  const unsigned nblk = (unsigned)((pr.P + 63) / 64);
+  const int64_t tiles = (pr.P + 63) / 64;
+  const unsigned nblk = (unsigned)(tl.split0 + (tiles - tl.split0) * tl.slices);
   hipLaunchKernelGGL((shade_backward_kernel<V, DIFF, SPEC>), dim3(nblk), dim3(64 * tl.ntc),
                      tl.lds_bwd, stream, pr, tl);
   return hipGetLastError();
@@ -913,14 +975,51 @@ hipError_t launch_backward(int variant, bool diff, bool spec, const Problem& pr,
   return e;
 }
 
-hipError_t launch_envgrad_reduce(const Problem& pr, const Tiling& tl, hipStream_t stream) {
+namespace {
+
+template <int V, bool DIFF, bool SPEC>
+hipError_t launch_stage1(const Problem& pr, const Tiling& tl, hipStream_t stream) {
   const int ncol = 3 * pr.T;
   const int64_t nrows = (pr.P + 63) / 64;
   const int ngroups = (int)((nrows + tl.reduce_rows - 1) / tl.reduce_rows);
+  const int ncomb = (int)(((nrows - tl.split0) * 64 + 255) / 256);
   const unsigned cb = (unsigned)((ncol + 255) / 256);
-  hipLaunchKernelGGL(reduce_partials_kernel, dim3(cb, ngroups), dim3(256), 0, stream, pr.part,
-                     pr.part2, nrows, ncol, tl.reduce_rows);
-  hipError_t e = hipGetLastError();
+  hipLaunchKernelGGL((reduce_partials_kernel<V, DIFF, SPEC>), dim3(cb, ngroups + ncomb), dim3(256),
+                     0, stream, pr, tl, nrows, ncol, tl.reduce_rows, ngroups);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_envgrad_reduce(int variant, bool diff, bool spec, const Problem& pr,
+                                 const Tiling& tl, hipStream_t stream) {
+  const int ncol = 3 * pr.T;
+  const int64_t nrows = (pr.P + 63) / 64;
+  const int ngroups = (int)((nrows + tl.reduce_rows - 1) / tl.reduce_rows);
+  if (tl.split0 > nrows || tl.slices < 1) return hipErrorInvalidValue;
+  hipError_t e;
+  switch (variant) {
+    case kDiffuse: e = launch_stage1<kDiffuse, true, false>(pr, tl, stream); break;
+    case kSpecular:
+      if (diff && spec) e = launch_stage1<kSpecular, true, true>(pr, tl, stream);
+      else if (diff) e = launch_stage1<kSpecular, true, false>(pr, tl, stream);
+      else if (spec) e = launch_stage1<kSpecular, false, true>(pr, tl, stream);
+      else e = launch_stage1<kSpecular, false, false>(pr, tl, stream);
+      break;
+    case kClamp:
+      if (diff && spec) e = launch_stage1<kClamp, true, true>(pr, tl, stream);
+      else if (diff) e = launch_stage1<kClamp, true, false>(pr, tl, stream);
+      else if (spec) e = launch_stage1<kClamp, false, true>(pr, tl, stream);
+      else e = launch_stage1<kClamp, false, false>(pr, tl, stream);
+      break;
+    case kV2:
+      e = spec ? launch_stage1<kV2, true, true>(pr, tl, stream) : launch_stage1<kV2, true, false>(pr, tl, stream);
+      break;
+    case kVis:
+      e = spec ? launch_stage1<kVis, true, true>(pr, tl, stream) : launch_stage1<kVis, true, false>(pr, tl, stream);
+      break;
+    default: return hipErrorInvalidValue;
+  }
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(finish_envgrad_kernel, dim3((unsigned)((ncol + 15) / 16)), dim3(256), 0,
                      stream, pr.part2, pr.gE, ngroups, pr.H, pr.W);

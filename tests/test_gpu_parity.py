@@ -27,6 +27,7 @@ VARIANT_FLAGS = [
     ("specular", True, True), ("specular", True, False), ("specular", False, True),
     ("specular", False, False),
     ("clamp", True, True), ("clamp", True, False), ("clamp", False, True),
+    ("clamp", False, False),
     ("v2", True, True), ("v2", True, False),
     ("vis", True, True), ("vis", True, False),
 ]
@@ -151,3 +152,36 @@ def test_tiling_override_consistent():
 def test_sharp_lobes_grazing_views(variant, diff, spec):
     d = stress_problem(seed=7)
     compare(variant, gpu_run(variant, d, diff, spec), oracle_run(variant, d, diff, spec))
+
+
+TAIL_VARIANTS = [("vis", True, True), ("v2", True, True), ("specular", True, False),
+                 ("clamp", False, True), ("diffuse", True, False)]
+
+
+@pytest.mark.parametrize("variant,diff,spec", TAIL_VARIANTS)
+@pytest.mark.parametrize("H,W", [(8, 16), (5, 32)])
+def test_backward_tail_slices(variant, diff, spec, H, W):
+    # 66 000 points = 1032 tiles: the 8 beyond the 1024 resident tiles are
+    # shaded as four texel slices each and combined after the launch (on a
+    # 5 x 32 map the slices are 64, 64, 32 and 0 texels)
+    d = problem(seed=7, n=66000, H=H, W=W, vis="identity" if variant == "vis" else "none")
+    compare(variant, gpu_run(variant, d, diff, spec), oracle_run(variant, d, diff, spec))
+
+
+def test_tail_slices_change_results_by_rounding_only():
+    # the same problem with whole tiles only (a launch-shape override turns the
+    # slicing off): the outputs agree to fp32 rounding, and each way is
+    # reproducible bit for bit
+    from diff_envshade_amd import _lib
+    d = problem(seed=8, n=70000, H=16, W=32, vis="identity", n_bright=3)
+    a = gpu_run("vis", d)
+    b = gpu_run("vis", d)
+    for x, y in zip(a[0] + a[1], b[0] + b[1]):
+        assert np.array_equal(x, y)
+    try:
+        _lib.set_tiling_override(0, 4)
+        c = gpu_run("vis", d)
+    finally:
+        _lib.set_tiling_override(0, 0)
+    for x, y in zip(a[0] + a[1], c[0] + c[1]):
+        assert relerr(x, y) <= 1e-6
