@@ -235,25 +235,54 @@ int launch_error(hipError_t e, const char* what) {
   return fail(DES_ERR_LAUNCH, "%s: %s", what, hipGetErrorString(e));
 }
 
+// Forward workspace (when the points are reordered): the point order, then
+// the per-point sums of the tail slices.
+struct FwdLayout {
+  size_t spart_off, total;
+};
+
+FwdLayout fwd_layout(int64_t P, int T) {
+  FwdLayout f;
+  f.spart_off = align256(des::order_layout(P).total);
+  const TailSplit ts = tail_split(P, T);
+  f.total = f.spart_off +
+            align256(size_t(n_tiles(P) - ts.split0) * ts.slices * des::kForwardAcc * 64 *
+                     sizeof(float));
+  return f;
+}
+
 int run_forward(int variant, bool diff, bool spec, des::Problem& pr, void* ws, size_t ws_bytes,
                 hipStream_t stream) {
   const void* reuse = des_capi::take_reuse_order(pr.P);
   if (pr.P == 0) return ok();
-  if (reuse != nullptr && use_order(pr.P, pr.T, false)) {
-    pr.order = reinterpret_cast<const int32_t*>(static_cast<const char*>(reuse) +
-                                                des::order_layout(pr.P).order_off);
-  } else if (ws != nullptr && use_order(pr.P, pr.T, false)) {
-    const des::OrderLayout ol = des::order_layout(pr.P);
-    if (ws_bytes < ol.total)
+  const bool ordered = use_order(pr.P, pr.T, false);
+  const bool use_ws = ws != nullptr && ordered;
+  const FwdLayout fl = fwd_layout(pr.P, pr.T);
+  if (use_ws) {
+    if (ws_bytes < fl.total)
       return fail(DES_ERR_WORKSPACE_TOO_SMALL, "workspace of %zu bytes is smaller than the %zu required",
-                  ws_bytes, ol.total);
+                  ws_bytes, fl.total);
     if ((reinterpret_cast<uintptr_t>(ws) & 255) != 0)
       return fail(DES_ERR_INVALID_ARGUMENT, "workspace must be 256-byte aligned");
+  }
+  if (reuse != nullptr && ordered) {
+    pr.order = reinterpret_cast<const int32_t*>(static_cast<const char*>(reuse) +
+                                                des::order_layout(pr.P).order_off);
+  } else if (use_ws) {
     hipError_t e = des::launch_point_order(pr.normal, pr.P, ws, stream);
     if (e != hipSuccess) return launch_error(e, "point order failed");
-    pr.order = reinterpret_cast<const int32_t*>(static_cast<char*>(ws) + ol.order_off);
+    pr.order = reinterpret_cast<const int32_t*>(static_cast<char*>(ws) +
+                                                des::order_layout(pr.P).order_off);
   }
-  const des::Tiling tl = choose_tiling(pr.P, pr.T, false);
+  des::Tiling tl = choose_tiling(pr.P, pr.T, false);
+  // tail slices need the workspace for their per-point sums
+  if (use_ws && tl.ntc == 4 && g_ntc_fwd_override.load() <= 0) {
+    const TailSplit ts = tail_split(pr.P, pr.T);
+    tl.split0 = ts.split0;
+    tl.slices = ts.slices;
+    tl.slice_texels = ts.slice_texels;
+    pr.spart = reinterpret_cast<float*>(static_cast<char*>(ws) + fl.spart_off);
+  }
   hipError_t e = des::launch_forward(variant, diff, spec, pr, tl, stream);
   if (e != hipSuccess) return launch_error(e, "forward launch failed");
   return ok();
@@ -520,7 +549,7 @@ DES_API size_t des_backward_workspace_size(int64_t P, int32_t H, int32_t W) {
 
 DES_API size_t des_forward_workspace_size(int64_t P, int32_t H, int32_t W) {
   if (P <= 0 || H <= 0 || W <= 0 || !use_order(P, H * W, false)) return 0;
-  return des::order_layout(P).total;
+  return fwd_layout(P, H * W).total;
 }
 
 DES_API int des_reuse_point_order_once(const void* order_workspace, int64_t P) {

@@ -355,11 +355,37 @@ __device__ __forceinline__ void forward_chunk(const PointState<V, SPEC>& st, con
 // writes the outputs.  No barrier follows the last one executed here, so a
 // caller reusing LDS must synchronise first.
 // order: point of each tile position (tile * 64 + lane), or null for input order.
+// Outputs of point p from its sums acc (kForwardAcc entries: sum E vis
+// max(l.n,0) and sum E vis F G D / 4 per channel).
+template <int V, bool DIFF, bool SPEC>
+__device__ __forceinline__ void write_forward_outputs(const Problem& pr, int64_t p,
+                                                      const float* acc) {
+  constexpr bool DOUT = (V == kV2 || V == kVis);
+  float dsh[3], out[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    dsh[c] = acc[c] * kInvPi;
+    float o = 0.f;
+    if (DIFF) o = pr.albedo[3 * p + c] * dsh[c];
+    if (SPEC) o = fmaf(pr.spec[3 * p + c], acc[3 + c], o);
+    out[c] = o;
+  }
+#pragma unroll
+  for (int c = 0; c < 3; ++c) pr.sh[3 * p + c] = out[c];
+  if (DOUT) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) pr.ds[3 * p + c] = dsh[c];
+  }
+}
+
+// Texels [t_begin, t_end) of one 64-point tile (t_begin a multiple of 32);
+// slice_item >= 0 marks a tail slice, whose per-point sums go to
+// pr.spart[slice_item] instead of the outputs.
 template <int V, bool DIFF, bool SPEC>
 __device__ __forceinline__ void forward_tile(const Problem& pr, const Tiling& tl, int64_t tile,
-                                             const int32_t* order) {
+                                             const int32_t* order, int t_begin, int t_end,
+                                             int64_t slice_item) {
   extern __shared__ float4 smem[];
-  constexpr bool DOUT = (V == kV2 || V == kVis);
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
   const int64_t q = tile * 64 + lane;
@@ -382,8 +408,8 @@ __device__ __forceinline__ void forward_tile(const Problem& pr, const Tiling& tl
 
   Texel* tab = reinterpret_cast<Texel*>(smem);
   uint32_t* svis = reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(smem) + tl.vis_off);
-  for (int s0 = 0; s0 < pr.T; s0 += tl.st) {
-    const int sn = min(tl.st, pr.T - s0);
+  for (int s0 = t_begin; s0 < t_end; s0 += tl.st) {
+    const int sn = min(tl.st, t_end - s0);
     __syncthreads();
     build_table(pr, s0, sn, min(tl.st, (sn + 31) & ~31), tab);
     stage_vis(pr, st, s0, sn, single, svis, wv, tl.ntc, lane);
@@ -409,23 +435,17 @@ __device__ __forceinline__ void forward_tile(const Problem& pr, const Tiling& tl
       acc[j] = sum;
     }
   }
+  if (slice_item >= 0) {
+    // a tail slice: its per-point sums, combined after the launch
+    float* o = pr.spart + (int64_t)slice_item * NACC * 64;
+#pragma unroll
+    for (int j = 0; j < NACC; ++j) o[j * 64 + lane] = acc[j];
+    return;
+  }
   if (!valid) return;
-  float dsh[3], out[3];
-#pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    dsh[c] = acc[c] * kInvPi;
-    float o = 0.f;
-    if (DIFF) o = pr.albedo[3 * p + c] * dsh[c];
-    if (SPEC) o = fmaf(pr.spec[3 * p + c], acc[3 + c], o);
-    out[c] = o;
-  }
-#pragma unroll
-  for (int c = 0; c < 3; ++c) pr.sh[3 * p + c] = out[c];
-  if (DOUT) {
-#pragma unroll
-    for (int c = 0; c < 3; ++c) pr.ds[3 * p + c] = dsh[c];
-  }
+  write_forward_outputs<V, DIFF, SPEC>(pr, p, acc);
 }
+
 
 // Per-point texel sums of the backward (one wave's share of the texels).
 struct BwdSums {
@@ -685,7 +705,40 @@ __device__ __forceinline__ void backward_tile(const Problem& pr, const Tiling& t
 
 template <int V, bool DIFF, bool SPEC>
 __global__ __launch_bounds__(1024) void shade_forward_kernel(Problem pr, Tiling tl) {
-  forward_tile<V, DIFF, SPEC>(pr, tl, blockIdx.x, pr.order);
+  const int64_t b = blockIdx.x;
+  if (b < tl.split0) {
+    forward_tile<V, DIFF, SPEC>(pr, tl, b, pr.order, 0, pr.T, -1);
+  } else {
+    // tail slices, dispatched after the whole tiles
+    const int64_t j = b - tl.split0;
+    const int64_t tile = tl.split0 + j / tl.slices;
+    const int t0 = (int)(j % tl.slices) * tl.slice_texels;
+    forward_tile<V, DIFF, SPEC>(pr, tl, tile, pr.order, t0, min(pr.T, t0 + tl.slice_texels), j);
+  }
+}
+
+// The forward's tail slices: per point the slices' sums in slice order, then
+// the point's outputs.
+template <int V, bool DIFF, bool SPEC>
+__global__ __launch_bounds__(256) void forward_combine_kernel(Problem pr, Tiling tl) {
+  const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t ntail = (pr.P + 63) / 64 - tl.split0;  // >= 0 (checked at launch)
+  if (idx >= ntail * 64) return;
+  const int64_t k = idx >> 6;
+  const int lane = (int)(idx & 63);
+  const int64_t q = (tl.split0 + k) * 64 + lane;
+  if (q >= pr.P) return;
+  const int64_t p = pr.order != nullptr ? (int64_t)pr.order[q] : q;
+  constexpr int NACC = kForwardAcc;
+  float acc[NACC];
+#pragma unroll
+  for (int j = 0; j < NACC; ++j) acc[j] = 0.f;
+  for (int sl = 0; sl < tl.slices; ++sl) {
+    const float* in = pr.spart + (k * tl.slices + sl) * NACC * 64;
+#pragma unroll
+    for (int j = 0; j < NACC; ++j) acc[j] += in[j * 64 + lane];
+  }
+  write_forward_outputs<V, DIFF, SPEC>(pr, p, acc);
 }
 
 // Four waves per SIMD (<= 128 VGPRs): the measured best for this kernel.
@@ -743,7 +796,8 @@ __global__ __launch_bounds__(512) void batch_forward_kernel(const BatchHeader* _
     if (g >= total) break;
     const int s = __builtin_amdgcn_readfirstlane(
         find_set(n, g, [&](int i) { return sets[i].tile0; }));
-    forward_tile<V, DIFF, SPEC>(sets[s].pr, sets[s].tl_fwd, g - sets[s].tile0, sets[s].order_fwd);
+    forward_tile<V, DIFF, SPEC>(sets[s].pr, sets[s].tl_fwd, g - sets[s].tile0, sets[s].order_fwd,
+                                0, sets[s].pr.T, -1);
   }
 }
 
@@ -906,9 +960,16 @@ __global__ __launch_bounds__(256) void fill_zero_kernel(float* __restrict__ x, i
 
 template <int V, bool DIFF, bool SPEC>
 hipError_t launch_fwd(const Problem& pr, const Tiling& tl, hipStream_t stream) {
-  const unsigned nblk = (unsigned)((pr.P + 63) / 64);
+  const int64_t tiles = (pr.P + 63) / 64;
+  if (tl.split0 > tiles || tl.slices < 1) return hipErrorInvalidValue;
+  const unsigned nblk = (unsigned)(tl.split0 + (tiles - tl.split0) * tl.slices);
   hipLaunchKernelGGL((shade_forward_kernel<V, DIFF, SPEC>), dim3(nblk), dim3(64 * tl.ntc),
                      tl.lds_fwd, stream, pr, tl);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess || tl.split0 == tiles) return e;
+  const unsigned ncomb = (unsigned)(((tiles - tl.split0) * 64 + 255) / 256);
+  hipLaunchKernelGGL((forward_combine_kernel<V, DIFF, SPEC>), dim3(ncomb), dim3(256), 0, stream,
+                     pr, tl);
   return hipGetLastError();
 }
 
