@@ -72,13 +72,27 @@ def _percentile(xs, q):
     return xs[f] + (xs[c] - xs[f]) * (k - f)
 
 
-def _errors(got, ref):
+def _errors(got, ref, kind=None):
+    """Normwise max and mean relative error; kind "color" adds the elementwise
+    relative error over |ref| >= 1e-3 max|ref|, kind "point" / "texel" the
+    per-point (per-texel) normwise error over the points whose gradient
+    vector is >= 1e-3 of the largest."""
     import numpy as np
     g = np.asarray(got, dtype=np.float64).reshape(np.shape(ref))
     r = np.asarray(ref, dtype=np.float64)
     mx = float(np.abs(g - r).max() / max(np.abs(r).max(), 1e-300))
     mean = float(np.abs(g - r).mean() / max(np.abs(r).mean(), 1e-300))
-    return {"max_rel": mx, "mean_rel": mean}
+    out = {"max_rel": mx, "mean_rel": mean}
+    if kind == "color":
+        m = np.abs(r) >= 1e-3 * np.abs(r).max()
+        out["elementwise_max_rel"] = float((np.abs(g - r)[m] / np.abs(r)[m]).max())
+    elif kind in ("point", "texel"):
+        rr = r.reshape(r.shape[0], -1) if kind == "point" else r.reshape(r.shape[0], -1).T
+        gg = g.reshape(g.shape[0], -1) if kind == "point" else g.reshape(g.shape[0], -1).T
+        rn = np.linalg.norm(rr, axis=1)
+        m = rn >= 1e-3 * rn.max()
+        out["per_point_max_rel"] = float((np.linalg.norm(gg - rr, axis=1)[m] / rn[m]).max())
+    return out
 
 
 def main() -> int:
@@ -263,14 +277,18 @@ def main() -> int:
             rsh, rds, rg = ref
             gn, gv, ga, gs, gr, gE = [x.cpu().numpy() for x in g]
             line["err_vs_fp64"] = {
-                "shading": _errors(sh.cpu().numpy(), rsh),
-                "diffuse_shading": _errors(ds.cpu().numpy(), rds),
-                "grad_normal": _errors(gn, rg["normal"]),
-                "grad_view_dir": _errors(gv, rg["view_dir"]),
-                "grad_albedo": _errors(ga, rg["albedo"]),
-                "grad_specular_albedo": _errors(gs, rg["specular_albedo"]),
-                "grad_roughness": _errors(gr, rg["roughness"]),
-                "grad_envmap": _errors(gE, rg["envmap"]),
+                "shading": _errors(sh.cpu().numpy(), rsh, "color"),
+                "diffuse_shading": _errors(ds.cpu().numpy(), rds, "color"),
+                "grad_normal": _errors(gn, rg["normal"], "point"),
+                "grad_view_dir": _errors(gv, rg["view_dir"], "point"),
+                "grad_albedo": _errors(ga, rg["albedo"], "point"),
+                "grad_specular_albedo": _errors(gs, rg["specular_albedo"], "point"),
+                "grad_roughness": _errors(gr, rg["roughness"], "point"),
+                "grad_envmap": _errors(gE, rg["envmap"], "texel"),
+                "note": "max_rel / mean_rel normwise (max|x-ref|/max|ref|, mean|x-ref|/mean|ref|); "
+                        "elementwise_max_rel over |ref| >= 1e-3 max|ref|; per_point_max_rel: "
+                        "per point (per texel for grad_envmap) |x_p-ref_p|/|ref_p| over the "
+                        "points >= 1e-3 of the largest; all on the timed run's outputs",
             }
         if not args.no_cpu_baseline:
             timed = sorted(runs[1:])

@@ -876,19 +876,20 @@ __device__ __forceinline__ void finish_item(const double* __restrict__ part2,
   __syncthreads();
 }
 
-// Stage 1, and in rows blockIdx.y >= ngroups (blockIdx.x == 0) the combine of
-// the backward's tail slices: per point the slices' sums in slice order, then
-// the point's gradients.
+// Stage 1, and in the first ncomb rows (blockIdx.x == 0) the combine of the
+// backward's tail slices: per point the slices' sums in slice order, then the
+// point's gradients.  The combine rows come first so that their chain of
+// dependent loads overlaps the reduction rows instead of trailing them.
 template <int V, bool DIFF, bool SPEC>
 __global__ __launch_bounds__(256) void reduce_partials_kernel(Problem pr, Tiling tl,
                                                               int64_t nrows, int ncol, int R,
-                                                              int ngroups) {
-  if ((int)blockIdx.y < ngroups) {
-    reduce_rows_item(pr.part, pr.part2, nrows, ncol, R, blockIdx.y, blockIdx.x);
+                                                              int ncomb) {
+  if ((int)blockIdx.y >= ncomb) {
+    reduce_rows_item(pr.part, pr.part2, nrows, ncol, R, blockIdx.y - ncomb, blockIdx.x);
     return;
   }
   if (blockIdx.x != 0) return;
-  const int64_t idx = (int64_t)(blockIdx.y - ngroups) * 256 + threadIdx.x;
+  const int64_t idx = (int64_t)blockIdx.y * 256 + threadIdx.x;
   const int64_t ntail = nrows - tl.split0;  // >= 0 (checked at launch)
   if (idx >= ntail * 64) return;
   const int64_t k = idx >> 6;
@@ -1046,7 +1047,7 @@ hipError_t launch_stage1(const Problem& pr, const Tiling& tl, hipStream_t stream
   const int ncomb = (int)(((nrows - tl.split0) * 64 + 255) / 256);
   const unsigned cb = (unsigned)((ncol + 255) / 256);
   hipLaunchKernelGGL((reduce_partials_kernel<V, DIFF, SPEC>), dim3(cb, ngroups + ncomb), dim3(256),
-                     0, stream, pr, tl, nrows, ncol, tl.reduce_rows, ngroups);
+                     0, stream, pr, tl, nrows, ncol, tl.reduce_rows, ncomb);
   return hipGetLastError();
 }
 

@@ -1,10 +1,15 @@
 """Shared helpers for the parity tests: run one variant through the HIP drop-in
 and through the FP64 oracle on the same inputs, and compare.
 
-Error metric: normwise relative error per output tensor,
-    err(x, ref) = max|x - ref| / max|ref|,
-which stays meaningful for signed sums such as grad_normal (elementwise
-relative error is unbounded near zero crossings).
+Error metrics, per output tensor:
+  * normwise relative error  err(x, ref) = max|x - ref| / max|ref|;
+  * for the colors (shading, diffuse_shading: sums of non-negative terms) the
+    elementwise relative error max |x - ref| / |ref| over the elements with
+    |ref| >= 1e-3 max|ref|, so that dim points are held to the same bound;
+  * for the gradients (signed sums, where elementwise relative error is
+    unbounded near zero crossings) the per-point normwise error
+    max_p |x_p - ref_p| / |ref_p| over the points (texels, for grad_envmap)
+    whose gradient vector has |ref_p| >= 1e-3 max_p |ref_p|.
 """
 from __future__ import annotations
 
@@ -15,6 +20,16 @@ from oracle import oracle as O
 
 # colors and every gradient must match the FP64 oracle to this normwise relative error
 TOL = 1e-5
+# the same bound for the elementwise error of the colors and the per-point
+# error of the gradients (over the elements / points above 1e-3 of the largest)
+TOL_LOCAL = 1e-5
+LOCAL_FLOOR = 1e-3
+# except grad_roughness: a scalar per point summing terms of both signs (the
+# GGX NDF's derivative in alpha^2 is negative near the lobe's peak and
+# positive away from it), so where those cancel its fp32 error relative to
+# its own size grows with the sum's conditioning (up to 4.8e-5 measured on
+# the parity cases)
+TOL_LOCAL_OVERRIDE = {"grad_roughness": 1e-4}
 
 FWD_NAMES = {"diffuse": ["shading"], "specular": ["shading"], "clamp": ["shading"],
              "v2": ["shading", "diffuse_shading"], "vis": ["shading", "diffuse_shading"]}
@@ -30,6 +45,41 @@ def relerr(x, ref) -> float:
     den = np.abs(ref).max() if ref.size else 0.0
     num = np.abs(x - ref).max() if ref.size else 0.0
     return float(num / den) if den > 0 else float(num)
+
+
+def elemerr(x, ref, floor=LOCAL_FLOOR) -> float:
+    """Elementwise relative error over |ref| >= floor * max|ref|."""
+    x = np.asarray(x, dtype=np.float64).reshape(np.shape(ref))
+    ref = np.asarray(ref, dtype=np.float64)
+    if ref.size == 0:
+        return 0.0
+    top = np.abs(ref).max()
+    if top == 0:
+        return float(np.abs(x).max()) if x.size else 0.0
+    m = np.abs(ref) >= floor * top
+    return float((np.abs(x - ref)[m] / np.abs(ref)[m]).max())
+
+
+def pointerr(x, ref, floor=LOCAL_FLOOR, texel_major=False) -> float:
+    """Per-point normwise error: rows of a P x k gradient (columns of a
+    3 x H x W env-map gradient, one per texel) as vectors, over the rows with
+    |ref_row| >= floor * max |ref_row|."""
+    ref = np.asarray(ref, dtype=np.float64)
+    x = np.asarray(x, dtype=np.float64).reshape(ref.shape)
+    if ref.size == 0:
+        return 0.0
+    if texel_major:
+        ref = ref.reshape(ref.shape[0], -1).T
+        x = x.reshape(x.shape[0], -1).T
+    else:
+        ref = ref.reshape(ref.shape[0], -1)
+        x = x.reshape(x.shape[0], -1)
+    rn = np.linalg.norm(ref, axis=1)
+    top = rn.max()
+    if top == 0:
+        return float(np.abs(x).max())
+    m = rn >= floor * top
+    return float((np.linalg.norm(x - ref, axis=1)[m] / rn[m]).max())
 
 
 def problem(seed=0, n=300, H=4, W=8, rotation=True, vis="none", p_vis=0.7, n_bright=0,
@@ -123,14 +173,23 @@ def gpu_run(variant, d, diff=True, spec=True, device="cuda"):
     return [x.cpu().numpy() for x in fwd], [x.cpu().numpy() for x in bwd]
 
 
-def compare(variant, got, ref, tol=TOL):
-    """Assert every output within tol; returns {name: err}."""
+def compare(variant, got, ref, tol=TOL, tol_local=TOL_LOCAL):
+    """Assert every output within tol (normwise) and tol_local (elementwise for
+    the colors, per point for the gradients); returns {name: normwise err}.
+    tol_local=None skips the local check."""
     (gf, gb), (rf, rb) = got, ref
-    errs = {}
+    errs, local = {}, {}
     for name, x, r in zip(FWD_NAMES[variant], gf, rf):
         errs[name] = relerr(x, r)
+        local[name] = elemerr(x, r)
     for name, x, r in zip(BWD_NAMES[variant], gb, rb):
         errs["grad_" + name] = relerr(x.reshape(r.shape), r)
+        local["grad_" + name] = pointerr(x, r, texel_major=(name == "envmap"))
     bad = {k: v for k, v in errs.items() if not v <= tol}
     assert not bad, f"{variant}: outputs beyond {tol}: {bad} (all: {errs})"
+    if tol_local is not None:
+        bad = {k: v for k, v in local.items()
+               if not v <= max(tol_local, TOL_LOCAL_OVERRIDE.get(k, 0.0))}
+        assert not bad, (f"{variant}: local errors beyond {tol_local}: {bad} (all: {local}; "
+                         f"normwise {errs})")
     return errs

@@ -20,7 +20,12 @@ built from the mathematics the reference implements
   * Cook-Torrance with Schlick F (F0 = 0.04), GGX D with alpha = r^2 and
     Smith-Schlick G1(x) = x / (x (1 - k) + k), k = (r + 1)^2 / 8:
     f = D F G1(n.l) G1(n.v) / (4 n.l n.v), radiance = E * s * f * n.l * dOmega
-    (bs:2218-2234, written there as F G D / 4 with G = G1 G1 / n.v).
+    (bs:2218-2234, written there as F G D / 4 with G = G1 G1 / n.v);
+  * a view at or below the horizon (n.v <= 0): the reference clamps n.v to 0
+    inside G (bs:161-171), so G = G1(n.l) / k -- the n.v -> 0+ limit of
+    G1(n.v) / n.v -- and the non-clamp variants still shade the specular
+    lobe, with F saturating to 1 where l.h < 0; the clamp variant skips the
+    whole point (bs:1111, 1125).
 
 Run:  python tests/golden/make_kat.py   (writes the JSON next to this file)
 """
@@ -80,32 +85,35 @@ def smith_G1(x, k):
     return x / (x * (1.0 - k) + k)
 
 
-def case(name, H, W, r, x, a_deg, b_deg, rough, E, albedo, spec, vscale=1.0):
+def case(name, H, W, r, x, a_deg, b_deg, rough, E, albedo, spec, vscale=1.0, view=None):
     a = math.radians(a_deg)
     b = math.radians(b_deg)
     d, dw = texel_geometry(H, W, r, x)
     l = np.array([math.sin(a), math.cos(a), 0.0])
     M = rotation_taking(d, l)
     n = np.array([0.0, 1.0, 0.0])
-    v = vscale * np.array([-math.sin(b), math.cos(b), 0.0])
+    v = vscale * np.array([-math.sin(b), math.cos(b), 0.0]) if view is None else np.array(view)
     alpha = rough * rough
     k = (rough + 1.0) ** 2 / 8.0
     cos_l, cos_v = math.cos(a), math.cos(b)
     cos_h = math.cos((a - b) / 2.0)
     cos_d = math.cos((a + b) / 2.0)
     diffuse = [E[c] * max(cos_l, 0.0) / math.pi * dw for c in range(3)]
+    nv = float(np.dot(n, v))
     if cos_l > 0:
         # the reference's G uses the unnormalised view vector's n.v; v is unit
         # here unless vscale != 1, in which case n.v and l.h change with |v|
-        nv = float(np.dot(n, v))
         h = (l + v) / np.linalg.norm(l + v)
         cos_h = float(np.dot(n, h))
-        cos_d = float(np.dot(l, h))
-        f = ggx_D(cos_h, alpha) * schlick_F(cos_d) * smith_G1(cos_l, k) * smith_G1(nv, k) \
-            / (4.0 * cos_l * nv)
+        cos_d = max(float(np.dot(l, h)), 0.0)  # Schlick takes sat(l.h)
+        # G1(n.v) / n.v, or its n.v -> 0+ limit 1/k where the reference clamps n.v to 0
+        g1v_over_nv = smith_G1(nv, k) / nv if nv > 0 else 1.0 / k
+        f = ggx_D(cos_h, alpha) * schlick_F(cos_d) * smith_G1(cos_l, k) * g1v_over_nv \
+            / (4.0 * cos_l)
         specular = [E[c] * spec[c] * f * cos_l * dw for c in range(3)]
     else:
         specular = [0.0, 0.0, 0.0]
+    shading = [albedo[c] * diffuse[c] + specular[c] for c in range(3)]
     env = np.zeros((3, H, W))
     env[:, r, x] = E
     t = r * W + x
@@ -116,8 +124,10 @@ def case(name, H, W, r, x, a_deg, b_deg, rough, E, albedo, spec, vscale=1.0):
         "light2obj_rotmat": M.tolist(),
         "expected": {
             "diffuse_shading": diffuse,
-            "shading_v2": [albedo[c] * diffuse[c] + specular[c] for c in range(3)],
+            "shading_v2": shading,
             "shading_specular_only": specular,
+            # clamp variant, both branches on: the point is skipped unless n.v > 0
+            "shading_clamp": shading if nv > 0 else [0.0, 0.0, 0.0],
         },
     }
 
@@ -138,6 +148,13 @@ def main():
              [0.6, 0.6, 0.6], [0.07, 0.03, 0.01], vscale=1.7),
         case("below_horizon", 8, 16, 2, 11, 120.0, 30.0, 0.5, [5.0, 5.0, 5.0],
              [1.0, 1.0, 1.0], [0.1, 0.1, 0.1]),
+        # views at and below the horizon (n.v <= 0)
+        case("back_facing_view", 8, 16, 2, 4, 30.0, 120.0, 0.4, [2.0, 1.0, 3.0],
+             [0.5, 0.5, 0.5], [0.1, 0.2, 0.05]),
+        case("view_on_horizon", 16, 32, 5, 20, 40.0, 90.0, 0.35, [1.0, 3.0, 2.0],
+             [0.2, 0.8, 0.4], [0.06, 0.06, 0.06], view=[-1.0, 0.0, 0.0]),
+        case("back_facing_l_dot_h_negative", 8, 16, 3, 13, 70.0, 150.0, 0.7, [4.0, 2.0, 1.0],
+             [0.9, 0.3, 0.6], [0.1, 0.1, 0.1]),
     ]
     with open(OUT, "w") as f:
         json.dump({"source": "tests/golden/make_kat.py", "cases": cases}, f, indent=1)

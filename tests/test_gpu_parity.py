@@ -185,3 +185,45 @@ def test_tail_slices_change_results_by_rounding_only():
         _lib.set_tiling_override(0, 0)
     for x, y in zip(a[0] + a[1], c[0] + c[1]):
         assert relerr(x, y) <= 1e-6
+
+
+def _kat_cases():
+    import json
+    import os
+    here = os.path.dirname(os.path.abspath(__file__))
+    with open(os.path.join(here, "golden", "kat_single_texel.json")) as f:
+        return json.load(f)["cases"]
+
+
+@pytest.mark.parametrize("c", _kat_cases(), ids=lambda c: c["name"])
+def test_known_answers_on_the_gpu(c):
+    # the single-texel known answers (textbook angle form, including views at
+    # and below the horizon) through the HIP forward of the v2, specular-only
+    # and clamp variants; inputs rounded to fp32, hence rtol 1e-5
+    import torch
+    from diff_envshade_amd import _diff_envshade as C
+    H, W = c["H"], c["W"]
+    env = np.zeros((3, H * W), dtype=np.float32)
+    env[:, c["texel"]] = c["E"]
+    f32 = lambda x: torch.tensor(np.asarray(x, dtype=np.float32), device="cuda")  # noqa: E731
+    mat = [f32([c["normal"]]), f32([c["view_dir"]]), f32([c["albedo"]]),
+           f32([c["specular_albedo"]]), f32([[c["roughness"]]]), f32(env.reshape(3, H, W)),
+           f32(c["light2obj_rotmat"])]
+    e = c["expected"]
+    sh, ds = C.bruteforce_specularvisibility_shader_forward2(
+        *mat[:7], torch.zeros((3, 1), dtype=torch.int32, device="cuda"),
+        torch.zeros(1, dtype=torch.int64, device="cuda"),
+        torch.tensor([[1.0], [0.0], [0.0]], device="cuda"),
+        torch.full(((H * W + 31) // 32, 1), -1, dtype=torch.int32, device="cuda"), True)
+    v2 = C.bruteforce_specular_shader_forward2(*mat, True)
+    sp = C.bruteforce_specular_shader_forward(*mat, False, True)
+    cl = C.bruteforce_specular_shader_clamp_forward(*mat, True, True)
+    torch.cuda.synchronize()
+    for got, want in ((sh, e["shading_v2"]), (ds, e["diffuse_shading"]), (v2[0], e["shading_v2"]),
+                      (v2[1], e["diffuse_shading"]), (sp, e["shading_specular_only"]),
+                      (cl, e["shading_clamp"])):
+        np.testing.assert_allclose(got.cpu().numpy()[0], want, rtol=1e-5, atol=1e-12)
+    if np.dot(c["normal"], c["view_dir"]) <= 0:
+        # a view at or below the horizon: specular in v1 / v2, nothing in clamp
+        assert (sp.cpu().numpy() > 0).all()
+        assert (cl.cpu().numpy() == 0).all()

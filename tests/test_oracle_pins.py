@@ -108,6 +108,12 @@ def test_known_answers(c):
     df = O.forward("diffuse", d)[0]
     np.testing.assert_allclose(df[0], np.array(c["albedo"]) * e["diffuse_shading"], rtol=1e-12,
                                atol=1e-15)
+    # the clamp variant skips a point unless n.v > 0; the others shade a
+    # view at or below the horizon with G = G1(n.l) / k (bs:161-171)
+    cl, _ = O.forward("clamp", d, enable_diffuse=True, enable_specular=True)
+    np.testing.assert_allclose(cl[0], e["shading_clamp"], rtol=1e-12, atol=1e-15)
+    v1, _ = O.forward("specular", d, enable_diffuse=True, enable_specular=True)
+    np.testing.assert_allclose(v1[0], e["shading_v2"], rtol=1e-12, atol=1e-15)
 
 
 def test_roughness_one_gives_flat_ndf():
@@ -296,3 +302,17 @@ def test_zero_visibility_gives_zero():
     g = O.backward("vis", d, d["grad_shading"], d["grad_diffuse_shading"])
     for v in g.values():
         assert np.all(v == 0)
+
+
+def test_back_facing_views_shade_specular_except_in_the_clamp_variant():
+    # the known answers with n.v <= 0 carry a non-zero specular term that the
+    # non-clamp variants reproduce and the clamp variant does not
+    cases = [c for c in _kat_cases() if np.dot(c["normal"], c["view_dir"]) <= 0]
+    assert len(cases) == 3
+    for c in cases:
+        d = kat_inputs(c)
+        assert min(c["expected"]["shading_specular_only"]) > 0
+        sp, _ = O.forward("specular", d, enable_diffuse=False, enable_specular=True)
+        assert (sp[0] > 0).all()
+        cl, _ = O.forward("clamp", d, enable_diffuse=False, enable_specular=True)
+        assert (cl[0] == 0).all()
