@@ -26,11 +26,15 @@ using des_capi::ok;
 size_t a256(size_t x) { return (x + 255) & ~size_t(255); }
 
 struct RelightLayout {
-  size_t geo, lfrag, pconst, abfrag, part, total;
+  size_t geo, lfrag, pconst, abfrag, part, order, vsort, total;
   int kc, tp, kp, ntg_max, nf, npc, ks_per_chunk;
 };
 
-RelightLayout relight_layout(int64_t P, int E, int H, int W, int prec, bool envgrad) {
+// With visibility the points are shaded in normal-sorted order (rows of 16 /
+// 32 points facing one way see whole chunks dark, which the kernels skip), so
+// the workspace also holds the order and the visibility words in that order.
+RelightLayout relight_layout(int64_t P, int E, int H, int W, int prec, bool envgrad,
+                             bool with_vis) {
   RelightLayout L{};
   const int T = H * W;
   L.kc = (T + 31) / 32;
@@ -57,6 +61,13 @@ RelightLayout relight_layout(int64_t P, int E, int H, int W, int prec, bool envg
     L.part = off;
     off += a256(size_t(L.npc) * L.tp * L.ntg_max * 16 * 4);
   }
+  L.order = L.vsort = off;
+  if (with_vis && P > 0) {
+    L.order = off;
+    off += a256(des::order_layout(P).total);
+    L.vsort = off;
+    off += a256(size_t(L.kc) * size_t(P) * sizeof(uint32_t));
+  }
   L.total = off;
   return L;
 }
@@ -78,6 +89,7 @@ int check_relight(int64_t P, int32_t E, int32_t H, int32_t W, int32_t N_slot, in
 
 RelightArgs make_args(const RelightLayout& L, int64_t P, int E, int H, int W, void* ws) {
   RelightArgs a{};
+  a.blocked = (H % 4 == 0 && W % 8 == 0) ? 1 : 0;
   a.P = P;
   a.H = H;
   a.W = W;
@@ -103,6 +115,22 @@ int launch_err(hipError_t e, const char* what) {
   return fail(DES_ERR_LAUNCH, "%s: %s", what, hipGetErrorString(e));
 }
 
+// With visibility: sort the points by normal direction and gather their
+// visibility words into that order; without, shade in input order.
+hipError_t prepare_order(const RelightLayout& L, RelightArgs& a, void* ws, hipStream_t s) {
+  a.order = nullptr;
+  a.vsort = a.vis;
+  if (a.vis == nullptr) return hipSuccess;
+  char* w = static_cast<char*>(ws);
+  hipError_t e = des::launch_point_order(a.normal, a.P, w + L.order, s);
+  if (e != hipSuccess) return e;
+  a.order = reinterpret_cast<const int32_t*>(w + L.order + des::order_layout(a.P).order_off);
+  uint32_t* vs = reinterpret_cast<uint32_t*>(w + L.vsort);
+  e = des::relight_permute_vis(a, a.kc, vs, s);
+  a.vsort = vs;
+  return e;
+}
+
 }  // namespace
 
 extern "C" {
@@ -110,7 +138,7 @@ extern "C" {
 DES_API size_t des_relight_workspace_size(int64_t P, int32_t E, int32_t H, int32_t W,
                                           int32_t precision, int with_envgrad) {
   if (P < 0 || E < 1 || H <= 0 || W <= 0 || precision < 0 || precision > 2) return 0;
-  return relight_layout(P, E, H, W, precision, with_envgrad != 0).total;
+  return relight_layout(P, E, H, W, precision, with_envgrad != 0, true).total;
 }
 
 DES_API int des_relight_forward(int64_t P, int32_t E, int32_t H, int32_t W, int32_t N_slot,
@@ -126,7 +154,7 @@ DES_API int des_relight_forward(int64_t P, int32_t E, int32_t H, int32_t W, int3
     return fail(DES_ERR_INVALID_ARGUMENT, "normal, albedo, envmaps, light2obj_rotmat and outputs must not be NULL");
   if (enable_specular && (!view_dir || !specular_albedo || !roughness))
     return fail(DES_ERR_INVALID_ARGUMENT, "view_dir, specular_albedo and roughness must not be NULL");
-  const RelightLayout L = relight_layout(P, E, H, W, precision, false);
+  const RelightLayout L = relight_layout(P, E, H, W, precision, false, visibility != nullptr);
   if (!workspace || workspace_bytes < L.total)
     return fail(DES_ERR_WORKSPACE_TOO_SMALL, "workspace of %zu bytes is smaller than the %zu required",
                 workspace_bytes, L.total);
@@ -139,7 +167,9 @@ DES_API int des_relight_forward(int64_t P, int32_t E, int32_t H, int32_t W, int3
   a.vis = reinterpret_cast<const uint32_t*>(visibility);
   a.sh = shading; a.ds = diffuse_shading;
   const bool spec = enable_specular != 0;
-  hipError_t e = des::relight_prepare_texels(a, s);
+  hipError_t e = prepare_order(L, a, workspace, s);
+  if (e != hipSuccess) return launch_err(e, "point order");
+  e = des::relight_prepare_texels(a, s);
   if (e != hipSuccess) return launch_err(e, "texel table");
   for (int col0 = 0; col0 < a.ncol; col0 += des::kRelightMaxNT * 16) {
     a.col0 = col0;
@@ -169,7 +199,7 @@ DES_API int des_relight_backward_envmap(int64_t P, int32_t E, int32_t H, int32_t
     return fail(DES_ERR_INVALID_ARGUMENT, "normal, albedo, light2obj_rotmat and upstream gradients must not be NULL");
   if (enable_specular && (!view_dir || !specular_albedo || !roughness))
     return fail(DES_ERR_INVALID_ARGUMENT, "view_dir, specular_albedo and roughness must not be NULL");
-  const RelightLayout L = relight_layout(P, E, H, W, precision, true);
+  const RelightLayout L = relight_layout(P, E, H, W, precision, true, visibility != nullptr);
   if (!workspace || workspace_bytes < L.total)
     return fail(DES_ERR_WORKSPACE_TOO_SMALL, "workspace of %zu bytes is smaller than the %zu required",
                 workspace_bytes, L.total);
@@ -181,7 +211,9 @@ DES_API int des_relight_backward_envmap(int64_t P, int32_t E, int32_t H, int32_t
   a.vis = reinterpret_cast<const uint32_t*>(visibility);
   a.g_sh = grad_shading; a.g_ds = grad_diffuse_shading; a.gE = grad_envmaps;
   const bool spec = enable_specular != 0;
-  hipError_t e = des::relight_prepare_texels(a, s);
+  hipError_t e = prepare_order(L, a, workspace, s);
+  if (e != hipSuccess) return launch_err(e, "point order");
+  e = des::relight_prepare_texels(a, s);
   if (e != hipSuccess) return launch_err(e, "texel table");
   if ((e = des::relight_prepare_points(spec, a, s)) != hipSuccess) return launch_err(e, "point constants");
   for (int col0 = 0; col0 < a.ncol; col0 += des::kRelightMaxNT * 16) {

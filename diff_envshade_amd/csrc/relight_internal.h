@@ -46,6 +46,9 @@ struct RelightArgs {
   const float* envmaps;   // E x 3 x H x W
   const float* rotmat;
   const uint32_t* vis;    // N_slot x P (per-point words) or NULL = all visible
+  const int32_t* order;   // shading order: point of sorted position q, or NULL (input order)
+  const uint32_t* vsort;  // visibility words of sorted position q: [slot][q] (vis if unsorted)
+  int blocked;            // K chunks are 4 x 8 texel blocks of the map (else 32-texel runs)
   const float* g_sh;      // P x E x 3
   const float* g_ds;      // P x E x 3
   float* sh;              // P x E x 3
@@ -59,7 +62,25 @@ struct RelightArgs {
   float* part;            // [npc][tp][ntg*16]
 };
 
+// Chunk layout.  The MFMA's K dimension runs over 32-texel chunks in any
+// order of texels; with H % 4 == 0 and W % 8 == 0 chunk c is a 4 x 8 block of
+// the map (rows 4 br .. 4 br + 3, columns 8 bc .. 8 bc + 7), entry j at row
+// 4 br + j / 8, column 8 bc + j % 8, so that the chunk's directions are
+// compact and more often dark for a row of normal-sorted points; otherwise
+// chunk c is texels 32 c .. 32 c + 31.  Either way entries 8 g .. 8 g + 7 are
+// consecutive storage texels inside one 32-texel visibility slot.
+__host__ __device__ inline int chunk_texel(const RelightArgs& a, int c, int j) {
+  if (a.blocked) {
+    const int nbc = a.W >> 3;
+    const int br = c / nbc, bc = c - br * nbc;
+    return (4 * br + (j >> 3)) * a.W + 8 * bc + (j & 7);
+  }
+  return 32 * c + j;
+}
+
 hipError_t relight_prepare_texels(const RelightArgs& a, hipStream_t s);
+// visibility words in shading order: vsort[slot][q] = vis[slot][order[q]]
+hipError_t relight_permute_vis(const RelightArgs& a, int nslot, uint32_t* vsort, hipStream_t s);
 hipError_t relight_prepare_L(int prec, const RelightArgs& a, hipStream_t s);
 hipError_t relight_launch_forward(int prec, bool spec, const RelightArgs& a, hipStream_t s);
 hipError_t relight_prepare_points(bool spec, const RelightArgs& a, hipStream_t s);

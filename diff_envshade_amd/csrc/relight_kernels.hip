@@ -77,28 +77,55 @@ __device__ __forceinline__ PointGeom load_point_consts(const RelightArgs& a, int
   return point_geom(nx, ny, nz, vx, vy, vz, r);
 }
 
-template <bool SPEC>
+// The weights of one pair.  The split precision carries ~16 bits of each
+// weight and takes them from the FP64-geometry lobe the drop-in uses; bf16 and
+// fp16 keep 8 / 11 bits, for which the fp32 lobe is accurate to a few ulps.
+template <int PREC, bool SPEC>
 __device__ __forceinline__ void pair_weights(const PointGeom& pc, const Texel& g, float vis,
                                              float& w1, float& w2) {
-  const Lobe lb = eval_lobe<SPEC>(pc, g);
-  w1 = lb.nl * vis;
-  w2 = SPEC ? lb.tmp * vis : 0.f;
+  if constexpr (PREC == kPrecBF16x3) {
+    const Lobe lb = eval_lobe<SPEC>(pc, g);
+    w1 = lb.nl * vis;
+    w2 = SPEC ? lb.tmp * vis : 0.f;
+  } else {
+    eval_weights_f32<SPEC>(pc, g, w1, w2);
+    w1 *= vis;
+    w2 *= vis;
+  }
 }
 
 // ---- preparation kernels ----------------------------------------------------------
 
-// texel table, zero beyond T (tp entries); the radiance fields are unused here
+// texel table in chunk order (entry i = chunk i / 32, entry i % 32), zero
+// beyond T (tp entries); the radiance fields are unused here
 __global__ __launch_bounds__(256) void relight_texels_kernel(RelightArgs a) {
-  const int t = blockIdx.x * 256 + threadIdx.x;
-  if (t >= a.tp) return;
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= a.tp) return;
   Texel g = zero_texel();
+  const int t = i < 32 * a.kc ? chunk_texel(a, i >> 5, i & 31) : a.T;
   if (t < a.T) {
     double M[9];
 #pragma unroll
-    for (int i = 0; i < 9; ++i) M[i] = (double)a.rotmat[i];
+    for (int k = 0; k < 9; ++k) M[k] = (double)a.rotmat[k];
     g = make_texel(texel_geometry(t, a.H, a.W, M, solid_angle_scale(a.H, a.W)), 0.f, 0.f, 0.f);
   }
-  a.geo[t] = g;
+  a.geo[i] = g;
+}
+
+// vsort[slot][q] = vis[slot][order[q]], thread per (slot, q).  Workgroups are
+// dealt to the 8 XCDs round-robin by id; the id -> (slot, q block) map keeps
+// every q block of one slot on one XCD, so each slot row (P words) is fetched
+// into one L2 and the random gathers of its words hit there.
+__global__ __launch_bounds__(256) void relight_permute_vis_kernel(const uint32_t* __restrict__ vis,
+                                                                  const int32_t* __restrict__ order,
+                                                                  int64_t P, int nslot, int nqb,
+                                                                  uint32_t* __restrict__ vsort) {
+  const int L = blockIdx.x;
+  const int k = L >> 3;
+  const int sl = (k / nqb) * 8 + (L & 7);
+  const int64_t q = (int64_t)(k % nqb) * 256 + threadIdx.x;
+  if (sl >= nslot || q >= P) return;
+  vsort[(int64_t)sl * P + q] = vis[(int64_t)sl * P + order[q]];
 }
 
 // L fragments: block (chunk c, n-tile n), lane l, j -> t = 32 c + 8 (l/16) + j,
@@ -111,7 +138,7 @@ __global__ __launch_bounds__(64) void relight_L_kernel(RelightArgs a) {
   float v[8];
 #pragma unroll
   for (int j = 0; j < 8; ++j) {
-    const int t = 32 * c + 8 * (l >> 4) + j;
+    const int t = chunk_texel(a, c, 8 * (l >> 4) + j);
     float x = 0.f;
     if (t < a.T && col < a.ncol) {
       const int iy = a.H - 1 - t / a.W;
@@ -127,10 +154,12 @@ __global__ __launch_bounds__(64) void relight_L_kernel(RelightArgs a) {
   if (a.nf == 2) dst[64 + l] = lo;
 }
 
+// per-point constants of shading position q (point order[q], or q)
 __global__ __launch_bounds__(256) void relight_points_kernel(RelightArgs a, bool spec) {
-  const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (p >= (int64_t)a.kp * 32) return;
-  a.pconst[p] = load_point_consts(a, p, spec);
+  const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (q >= (int64_t)a.kp * 32) return;
+  const int64_t p = (a.order != nullptr && q < a.P) ? (int64_t)a.order[q] : q;
+  a.pconst[q] = load_point_consts(a, p, spec);
 }
 
 // A / B fragments of the env-gradient GEMM: block (k-step ks, n-tile n), lane l,
@@ -144,9 +173,10 @@ __global__ __launch_bounds__(64) void relight_AB_kernel(RelightArgs a, bool spec
   float va[8], vb[8];
 #pragma unroll
   for (int j = 0; j < 8; ++j) {
-    const int64_t p = 32 * ks + 8 * (l >> 4) + j;
+    const int64_t q = 32 * ks + 8 * (l >> 4) + j;
     float A = 0.f, B = 0.f;
-    if (p < a.P && col < a.ncol) {
+    if (q < a.P && col < a.ncol) {
+      const int64_t p = a.order != nullptr ? (int64_t)a.order[q] : q;
       const float gs = a.g_sh[p * a.ncol + col];
       const float gd = a.g_ds[p * a.ncol + col];
       A = fmaf(gs, a.albedo[3 * p + ch], gd) * kInvPi;
@@ -175,9 +205,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void r
   __shared__ Texel s_geo[32];
   const int lane = threadIdx.x & 63;
   const int wv = threadIdx.x >> 6;
-  const int64_t p = (int64_t)blockIdx.x * 64 + wv * 16 + (lane & 15);
+  // shading position q; its point p (the rows of 16 positions are normal-sorted
+  // when a.order is set)
+  const int64_t q = (int64_t)blockIdx.x * 64 + wv * 16 + (lane & 15);
+  const bool valid = q < a.P;
+  const int64_t p = (a.order != nullptr && valid) ? (int64_t)a.order[q] : q;
   const PointGeom pc = load_point_consts(a, p, SPEC);
-  const bool valid = p < a.P;
   const int tb = 8 * (lane >> 4);
 
   f32x4 acc1[kRelightMaxNT], acc2[kRelightMaxNT];
@@ -193,13 +226,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void r
     for (int i = threadIdx.x; i < nfrag; i += 256) s_frag[i] = src[i];
     if (threadIdx.x < 32) s_geo[threadIdx.x] = a.geo[32 * c + threadIdx.x];
     __syncthreads();
-    uint32_t word = valid ? 0xffffffffu : 0u;
-    if (a.vis != nullptr && valid) word = a.vis[(int64_t)c * a.P + p];
+    // the visibility of the lane's 8 texels (consecutive bits of one word)
+    uint32_t bits = valid ? 0xffu : 0u;
+    if (a.vis != nullptr && valid) {
+      const int t0 = chunk_texel(a, c, tb);
+      bits = (a.vsort[(int64_t)(t0 >> 5) * a.P + q] >> (t0 & 31)) & 0xffu;
+    }
+    // a chunk that the 16 points of this wave all see none of adds exactly 0
+    if (!__any(bits != 0u)) continue;
     float w1[8], w2[8];
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
-      const float vis = (float)__builtin_amdgcn_ubfe(word, tb + j, 1);
-      pair_weights<SPEC>(pc, s_geo[tb + j], vis, w1[j], w2[j]);
+      const float vis = (float)__builtin_amdgcn_ubfe(bits, j, 1);
+      pair_weights<PREC, SPEC>(pc, s_geo[tb + j], vis, w1[j], w2[j]);
       // one pair at a time: interleaving the eight lobes would hold all their
       // temporaries at once and cost occupancy
       __builtin_amdgcn_sched_barrier(0);
@@ -225,12 +264,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void r
       }
     }
   }
-  // epilogue: D[point row][column]; outputs are P x E x 3 (column = 3 e + channel)
-  const int64_t prow0 = (int64_t)blockIdx.x * 64 + wv * 16 + 4 * (lane >> 4);
+  // epilogue: D[position row][column]; outputs are P x E x 3 (column = 3 e + channel)
+  const int64_t qrow0 = (int64_t)blockIdx.x * 64 + wv * 16 + 4 * (lane >> 4);
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
-    const int64_t pp = prow0 + i;
-    if (pp >= a.P) continue;
+    const int64_t qq = qrow0 + i;
+    if (qq >= a.P) continue;
+    const int64_t pp = a.order != nullptr ? (int64_t)a.order[qq] : qq;
     float al[3] = {0.f, 0.f, 0.f}, sp[3] = {0.f, 0.f, 0.f};
 #pragma unroll
     for (int ch = 0; ch < 3; ++ch) {
@@ -260,44 +300,66 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void r
   constexpr int NF = (PREC == kPrecBF16x3) ? 2 : 1;
   __shared__ uint4 s_ab[kRelightMaxNT * 2 * NF * 64];
   __shared__ PointGeom s_pc[32];
-  __shared__ uint32_t s_vis[2][32];
+  __shared__ uint32_t s_vis[8][32];  // [8-texel group of the 2 chunks][position]
   const int lane = threadIdx.x & 63;
   const int wv = threadIdx.x >> 6;
-  const int t = (blockIdx.x * 4 + wv) * 16 + (lane & 15);  // < tp
-  const Texel g = a.geo[t];
-  const int slot_local = (t >> 5) - 2 * (int)blockIdx.x;    // 0 or 1
-  const int tbit = t & 31;
+  // Workgroup id -> (texel block bx, point chunk by): the workgroups of one
+  // point chunk read the same fragments of A and B, so they are kept on one
+  // XCD (ids are dealt to the 8 XCDs round-robin) and share its L2.
+  const int tblocks = a.tp / kRelightTexBlock;
+  const int k8 = (int)blockIdx.x >> 3;
+  const int by = (k8 / tblocks) * 8 + ((int)blockIdx.x & 7);
+  const int bx = k8 % tblocks;
+  if (by >= a.npc) return;
+  const int i = (bx * 4 + wv) * 16 + (lane & 15);  // chunk-order texel, < tp
+  const Texel g = a.geo[i];
+  const int cg = (((i >> 5) - 2 * bx) << 2) | ((i & 31) >> 3);  // 0..7
+  const int tbit = (chunk_texel(a, i >> 5, i & 24) & 31) + (i & 7);
   const int pb = 8 * (lane >> 4);
 
   f32x4 acc[kRelightMaxNT];
 #pragma unroll
   for (int n = 0; n < kRelightMaxNT; ++n) acc[n] = f32x4{0.f, 0.f, 0.f, 0.f};
 
-  const int ks0 = blockIdx.y * a.ks_per_chunk;
+  const int ks0 = by * a.ks_per_chunk;
   const int ks1 = min(a.kp, ks0 + a.ks_per_chunk);
   const int nab = a.ntg * 2 * NF * 64;
   for (int ks = ks0; ks < ks1; ++ks) {
     __syncthreads();
-    const uint4* src = a.abfrag + (int64_t)ks * nab;
-    for (int i = threadIdx.x; i < nab; i += 256) s_ab[i] = src[i];
-    if (threadIdx.x < 32) s_pc[threadIdx.x] = a.pconst[(int64_t)ks * 32 + threadIdx.x];
-    if (threadIdx.x >= 128 && threadIdx.x < 192) {
-      const int q = threadIdx.x - 128;  // (slot, point)
-      const int sl = q >> 5, pt = q & 31;
-      const int64_t p = (int64_t)ks * 32 + pt;
-      const int slot = 2 * (int)blockIdx.x + sl;
+    {
+      // word of 8-texel group (chunk 2 bx + cgi / 4, group cgi % 4) for position pt
+      const int cgi = threadIdx.x >> 5, pt = threadIdx.x & 31;
+      const int64_t q = (int64_t)ks * 32 + pt;
+      const int c = 2 * bx + (cgi >> 2);
       uint32_t w = 0u;
-      if (p < a.P && 32 * slot < a.T) w = (a.vis != nullptr) ? a.vis[(int64_t)slot * a.P + p] : 0xffffffffu;
-      s_vis[sl][pt] = w;
+      if (q < a.P && c < a.kc) {
+        const int t0 = chunk_texel(a, c, (cgi & 3) * 8);
+        if (t0 < a.T) w = (a.vis != nullptr) ? a.vsort[(int64_t)(t0 >> 5) * a.P + q] : 0xffffffffu;
+      }
+      s_vis[cgi][pt] = w;
     }
     __syncthreads();
+    // the visibility of this texel for the lane's 8 positions
+    uint32_t bits = 0u;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) bits |= ((s_vis[cg][pb + j] >> tbit) & 1u) << j;
+    // 64 texels that none of the 32 positions sees add exactly 0: skip the
+    // k-step before staging its fragments (the barrier also orders the s_vis
+    // reads above before the next k-step overwrites it)
+    if (!__syncthreads_or(bits != 0u)) continue;
+    const uint4* src = a.abfrag + (int64_t)ks * nab;
+    for (int i2 = threadIdx.x; i2 < nab; i2 += 256) s_ab[i2] = src[i2];
+    if (threadIdx.x < 32) s_pc[threadIdx.x] = a.pconst[(int64_t)ks * 32 + threadIdx.x];
+    __syncthreads();
+    // 16 texels that none of the 32 positions sees add exactly 0
+    if (!__any(bits != 0u)) continue;
     float w1[8], w2[8];
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
       const int pt = pb + j;
       const PointGeom& pc = s_pc[pt];
-      const float vis = (float)__builtin_amdgcn_ubfe(s_vis[slot_local][pt], tbit, 1);
-      pair_weights<SPEC>(pc, g, vis, w1[j], w2[j]);
+      const float vis = (float)__builtin_amdgcn_ubfe(bits, j, 1);
+      pair_weights<PREC, SPEC>(pc, g, vis, w1[j], w2[j]);
       __builtin_amdgcn_sched_barrier(0);  // one pair at a time, as in the forward
     }
     uint4 a1h, a1l, a2h, a2l;
@@ -328,8 +390,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void r
   }
   // epilogue: D[texel row][column] -> part[chunk][t][column]
   const int nc = a.ntg * 16;
-  float* dst = a.part + (int64_t)blockIdx.y * a.tp * nc;
-  const int trow0 = (blockIdx.x * 4 + wv) * 16 + 4 * (lane >> 4);
+  float* dst = a.part + (int64_t)by * a.tp * nc;
+  const int trow0 = (bx * 4 + wv) * 16 + 4 * (lane >> 4);
 #pragma unroll
   for (int n = 0; n < kRelightMaxNT; ++n) {
     if (n < a.ntg) {
@@ -343,13 +405,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void r
 __global__ __launch_bounds__(256) void relight_reduce_kernel(RelightArgs a) {
   const int nc = a.ntg * 16;
   const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (idx >= (int64_t)a.T * nc) return;
-  const int t = (int)(idx / nc);
+  if (idx >= (int64_t)(32 * a.kc) * nc) return;
+  const int i = (int)(idx / nc);  // chunk-order row
   const int cl = (int)(idx % nc);
   const int col = a.col0 + cl;
   if (col >= a.ncol) return;
+  const int t = chunk_texel(a, i >> 5, i & 31);
+  if (t >= a.T) return;
   double s = 0.0;
-  for (int k = 0; k < a.npc; ++k) s += (double)a.part[((int64_t)k * a.tp + t) * nc + cl];
+  for (int k = 0; k < a.npc; ++k) s += (double)a.part[((int64_t)k * a.tp + i) * nc + cl];
   const int iy = a.H - 1 - t / a.W;
   const double dw = solid_angle_scale(a.H, a.W) * sinpi(((double)iy + 0.5) / (double)a.H);
   a.gE[(int64_t)col * a.T + t] = (float)(s * dw);
@@ -367,7 +431,9 @@ hipError_t launch_fwd_p(bool spec, const RelightArgs& a, hipStream_t s) {
 
 template <int PREC>
 hipError_t launch_eg_p(bool spec, const RelightArgs& a, hipStream_t s) {
-  const dim3 grid((unsigned)(a.tp / kRelightTexBlock), (unsigned)a.npc);
+  // 1-D: (texel block, point chunk) from the id, the point chunks rounded up to
+  // a multiple of the 8 XCDs (see relight_envgrad_kernel)
+  const dim3 grid((unsigned)(a.tp / kRelightTexBlock * ((a.npc + 7) / 8) * 8));
   if (spec)
     hipLaunchKernelGGL((relight_envgrad_kernel<PREC, true>), grid, dim3(256), 0, s, a);
   else
@@ -430,8 +496,20 @@ hipError_t relight_launch_envgrad(int prec, bool spec, const RelightArgs& a, hip
 }
 
 hipError_t relight_reduce_envgrad(const RelightArgs& a, hipStream_t s) {
-  const int64_t n = (int64_t)a.T * a.ntg * 16;
+  const int64_t n = (int64_t)(32 * a.kc) * a.ntg * 16;
   hipLaunchKernelGGL(relight_reduce_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+}  // namespace des
+
+namespace des {
+
+hipError_t relight_permute_vis(const RelightArgs& a, int nslot, uint32_t* vsort, hipStream_t s) {
+  const int nqb = (int)((a.P + 255) / 256);
+  const unsigned grid = (unsigned)(nqb * ((nslot + 7) / 8) * 8);
+  hipLaunchKernelGGL(relight_permute_vis_kernel, dim3(grid), dim3(256), 0, s, a.vis, a.order, a.P,
+                     nslot, nqb, vsort);
   return hipGetLastError();
 }
 

@@ -177,6 +177,7 @@ struct PointGeom {
   float nx, ny, nz, vx, vy, vz;
   double nxd, nyd, nzd, vxd, vyd, vzd;
   float nv, k, omk, ib4, a2, a2pi, c4;
+  float n2c;  // 1 - |n|^2 (formed in FP64), for eval_weights_f32
 };
 
 __device__ __forceinline__ PointGeom point_geom(float nx, float ny, float nz, float vx, float vy,
@@ -195,7 +196,43 @@ __device__ __forceinline__ PointGeom point_geom(float nx, float ny, float nz, fl
   c.k = (r + 1.f) * (r + 1.f) * 0.125f;
   c.omk = 1.f - c.k;
   c.ib4 = 0.25f / (fmaxf(c.nv, 0.f) * c.omk + c.k);
+  c.n2c = (float)(1.0 - (c.nxd * c.nxd + c.nyd * c.nyd + c.nzd * c.nzd));
   return c;
+}
+
+// The two weights of a pair in fp32 only, for consumers that round them to
+// 16 bits (the bf16 / fp16 relighting): w1 = max(l.n, 0) and w2 = F G D / 4
+// (0 where the reference skips the lobe).  1 - (n.h)^2 comes from the
+// Lagrange identity |hb|^2 - (n.hb)^2 = |n x hb|^2 + (1 - |n|^2) |hb|^2, the
+// cross product formed with fused multiply-adds, so the NDF denominator keeps
+// a few ulps of relative accuracy at the peak of sharp lobes instead of the
+// cancellation of the reference's 1 - (n.h)^2.
+template <bool SPEC>
+__device__ __forceinline__ void eval_weights_f32(const PointGeom& s, const Texel& g, float& w1,
+                                                 float& w2) {
+  const float cs = fmaf(g.lz, s.nz, fmaf(g.ly, s.ny, g.lx * s.nx));
+  w1 = fmaxf(cs, 0.f);
+  w2 = 0.f;
+  if (!SPEC) return;
+  const float hx = g.lx + s.vx, hy = g.ly + s.vy, hz = g.lz + s.vz;
+  const float hb2 = fmaf(hz, hz, fmaf(hy, hy, hx * hx));
+  const float nhb = fmaf(s.nz, hz, fmaf(s.ny, hy, s.nx * hx));
+  const float cx = fmaf(s.ny, hz, -s.nz * hy);
+  const float cy = fmaf(s.nz, hx, -s.nx * hz);
+  const float cz = fmaf(s.nx, hy, -s.ny * hx);
+  const float sn2 = fmaf(s.n2c, hb2, fmaf(cz, cz, fmaf(cy, cy, cx * cx)));
+  const float rs = rsq(fmaxf(hb2, 1e-20f));
+  const float rs2 = rs * rs;
+  const float ch = fmaf(g.lz, hz, fmaf(g.ly, hy, g.lx * hx)) * rs;
+  const float nh = nhb * rs;
+  const float om = 1.f - sat(ch);
+  const float om2 = om * om;
+  const float F = fmaf(om2 * om2 * om, 1.f - kF0, kF0);
+  const float Gq = w1 * rcp(fmaf(w1, s.omk, s.k)) * s.ib4;
+  const float tD = fmaf(nh * nh, s.a2, sn2 * rs2);
+  const float D = s.a2pi * rcp(fmaxf(tD * tD, kThresOverPi));
+  const float tmp = F * Gq * D;
+  w2 = (tmp > 0.f && hb2 > 0.f) ? tmp : 0.f;
 }
 
 }  // namespace math
