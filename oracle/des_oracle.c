@@ -141,9 +141,9 @@ static inline double g1_robust(v3 l, v3 v, v3 n, double k) {
     double a = dln / (dln * (1.0 - k) + k);
     return a / (dvn * (1.0 - k) + k);
 }
-/* bs:173-201 */
+/* bs:173-201; gk_abs accumulates the magnitudes of the two terms added to gk */
 static inline void grad_g1_robust(v3 l, v3 v, v3 n, double k, double g,
-                                  v3* gl, v3* gv, v3* gn, double* gk) {
+                                  v3* gl, v3* gv, v3* gn, double* gk, double* gk_abs) {
     double dvn = dot3(v, n), dln = dot3(l, n);
     dvn = dvn > 0.0 ? dvn : 0.0;
     dln = dln > 0.0 ? dln : 0.0;
@@ -157,13 +157,16 @@ static inline void grad_g1_robust(v3 l, v3 v, v3 n, double k, double g,
         gl->x += gdl * n.x; gl->y += gdl * n.y; gl->z += gdl * n.z;
         gn->x += gdl * l.x; gn->y += gdl * l.y; gn->z += gdl * l.z;
     }
-    *gk += grad_a * dln * (dln - 1.0) / (a_det * a_det);
+    const double gk_l = grad_a * dln * (dln - 1.0) / (a_det * a_det);
+    *gk += gk_l;
     double gdv = grad_b * (1.0 - k);
     if (dvn > 0.0) {
         gv->x += gdv * n.x; gv->y += gdv * n.y; gv->z += gdv * n.z;
         gn->x += gdv * v.x; gn->y += gdv * v.y; gn->z += gdv * v.z;
     }
-    *gk += grad_b * (1.0 - dvn);
+    const double gk_v = grad_b * (1.0 - dvn);
+    *gk += gk_v;
+    *gk_abs += fabs(gk_l) + fabs(gk_v);
 }
 /* bs:203-210 */
 static inline double ndf1(v3 n, v3 h, double alpha2) {
@@ -172,15 +175,19 @@ static inline double ndf1(v3 n, v3 h, double alpha2) {
     double den = DES_PI * t * t;
     return alpha2 / (den > THRES ? den : THRES);
 }
-/* bs:212-227 */
-static inline void grad_ndf1(v3 n, v3 h, double alpha2, double g, v3* gn, v3* gh, double* ga2) {
+/* bs:212-227; ga2_abs accumulates the magnitudes of the terms added to ga2 */
+static inline void grad_ndf1(v3 n, v3 h, double alpha2, double g, v3* gn, v3* gh, double* ga2,
+                             double* ga2_abs) {
     double dnh = dot3(n, h);
     double t = dnh * dnh * (alpha2 - 1.0) + 1.0;
     double t2 = DES_PI * t * t;
-    *ga2 += g / (t2 > THRES ? t2 : THRES);
+    const double ga2_d = g / (t2 > THRES ? t2 : THRES);
+    *ga2 += ga2_d;
+    *ga2_abs += fabs(ga2_d);
     if (t2 > THRES) {
         double gt = (-2.0 * alpha2 * g) / (t2 * t);
         *ga2 += dnh * dnh * gt;
+        *ga2_abs += fabs(dnh * dnh * gt);
         double gdnh = 2.0 * gt * (alpha2 - 1.0) * dnh;
         gn->x += gdnh * h.x; gn->y += gdnh * h.y; gn->z += gdnh * h.z;
         gh->x += gdnh * n.x; gh->y += gdnh * n.y; gh->z += gdnh * n.z;
@@ -305,11 +312,15 @@ static void forward_point(const prob_t* q, int64_t p, double* sh, double* ds) {
 /*
  * Per-point backward (bs:2248-2506 and its v1/clamp/diffuse siblings) plus
  * the per-(point, texel) env-map gradient contribution (bs:2508-2683),
- * accumulated into gE_acc (3 x T).
+ * accumulated into gE_acc (3 x T).  gra_o, when given, receives the scale of
+ * grad_roughness: the magnitudes of the terms the reference adds into the
+ * alpha^2 and k gradients (two each per texel, bs:173-227), through the
+ * same chain factors as the gradient.  A finite-precision evaluation of
+ * those sums of terms of both signs errs in proportion to it.
  */
 static void backward_point(const prob_t* q, int64_t p, const double* gsh, const double* gds,
                            double* gn_o, double* gv_o, double* ga_o, double* gs_o, double* gr_o,
-                           double* gE_acc) {
+                           double* gra_o, double* gE_acc) {
     point_t pt;
     load_point(q, p, &pt);
     int T = q->H * q->W;
@@ -321,7 +332,8 @@ static void backward_point(const prob_t* q, int64_t p, const double* gsh, const 
     }
     v3 gn = {0, 0, 0}, gv = {0, 0, 0};
     double ga[3] = {0, 0, 0}, gsa[3] = {0, 0, 0};
-    double galpha2 = 0.0, gk = 0.0;
+    double galpha2 = 0.0, gk = 0.0, ga2_abs = 0.0, gk_abs = 0.0;
+    const double dr_da2 = 4.0 * pt.r * pt.r * pt.r, dr_dk = (1.0 + pt.r) / 4.0; /* bs:2501-2502 */
     if (!pt.culled) {
         for (int t = 0; t < T; t++) {
             v3 l = q->tl[t];
@@ -368,8 +380,8 @@ static void backward_point(const prob_t* q, int64_t p, const double* gsh, const 
                     v3 gh = {gch * l.x, gch * l.y, gch * l.z};
                     v3 gl = {gch * h.x, gch * h.y, gch * h.z}; /* discarded, as bs:2457 */
                     v3 gvp = {0, 0, 0}, gnp = {0, 0, 0};
-                    grad_g1_robust(l, pt.v, pt.n, pt.k, gG, &gl, &gvp, &gnp, &gk);
-                    grad_ndf1(pt.n, h, pt.alpha2, gD, &gnp, &gh, &galpha2);
+                    grad_g1_robust(l, pt.v, pt.n, pt.k, gG, &gl, &gvp, &gnp, &gk, &gk_abs);
+                    grad_ndf1(pt.n, h, pt.alpha2, gD, &gnp, &gh, &galpha2, &ga2_abs);
                     gh = grad_normalize3(hb, gh);
                     gvp = add3(gvp, gh);
                     gv = add3(gv, gvp);
@@ -387,8 +399,8 @@ static void backward_point(const prob_t* q, int64_t p, const double* gsh, const 
         if (gv_o) { gv_o[3 * p] = gv.x; gv_o[3 * p + 1] = gv.y; gv_o[3 * p + 2] = gv.z; }
         if (gs_o)
             for (int c = 0; c < 3; c++) gs_o[3 * p + c] = gsa[c];
-        /* bs:2501-2502 */
-        if (gr_o) gr_o[p] = galpha2 * 4.0 * pt.r * pt.r * pt.r + gk * (1.0 + pt.r) / 4.0;
+        if (gr_o) gr_o[p] = galpha2 * dr_da2 + gk * dr_dk;
+        if (gra_o) gra_o[p] = ga2_abs * dr_da2 + gk_abs * dr_dk;
     }
 }
 
@@ -462,14 +474,14 @@ int des_oracle_forward(int variant, int enable_diffuse, int enable_specular, int
  * zero-filled first, so disabled branches read back as zeros (bs:2889-2905,
  * bs:3205-3214).  Per-thread env-map partials are summed in thread order.
  */
-int des_oracle_backward(int variant, int enable_diffuse, int enable_specular, int64_t P, int H,
-                        int W, int N_face, int N_vertex, const double* grad_shading,
-                        const double* grad_diffuse_shading, const double* normal,
-                        const double* view_dir, const double* albedo, const double* spec_albedo,
-                        const double* rough, const double* envmap, const double* rotmat,
-                        const int32_t* fvl, const int64_t* tri, const double* bary,
-                        const uint32_t* vis, double* gn, double* gv, double* ga, double* gs,
-                        double* gr, double* gE, int nthreads) {
+int des_oracle_backward_ex(int variant, int enable_diffuse, int enable_specular, int64_t P,
+                           int H, int W, int N_face, int N_vertex, const double* grad_shading,
+                           const double* grad_diffuse_shading, const double* normal,
+                           const double* view_dir, const double* albedo, const double* spec_albedo,
+                           const double* rough, const double* envmap, const double* rotmat,
+                           const int32_t* fvl, const int64_t* tri, const double* bary,
+                           const uint32_t* vis, double* gn, double* gv, double* ga, double* gs,
+                           double* gr, double* gE, double* gr_abs, int nthreads) {
     prob_t q = {variant, enable_diffuse, enable_specular, P, H, W, N_face, N_vertex,
                 normal, view_dir, albedo, spec_albedo, rough, envmap, rotmat, fvl, tri, bary, vis,
                 NULL, NULL};
@@ -485,6 +497,7 @@ int des_oracle_backward(int variant, int enable_diffuse, int enable_specular, in
     if (ga) memset(ga, 0, sizeof(double) * 3 * (size_t)P);
     if (gs) memset(gs, 0, sizeof(double) * 3 * (size_t)P);
     if (gr) memset(gr, 0, sizeof(double) * (size_t)P);
+    if (gr_abs) memset(gr_abs, 0, sizeof(double) * (size_t)P);
     if (gE) memset(gE, 0, sizeof(double) * 3 * (size_t)T);
     int nt = des_oracle_max_threads();
     double* part = gE ? (double*)calloc((size_t)nt * 3 * T, sizeof(double)) : NULL;
@@ -498,7 +511,8 @@ int des_oracle_backward(int variant, int enable_diffuse, int enable_specular, in
         double* acc = part ? part + (size_t)tid * 3 * T : NULL;
 #pragma omp for schedule(static)
         for (int64_t p = 0; p < P; p++)
-            backward_point(&q, p, grad_shading, grad_diffuse_shading, gn, gv, ga, gs, gr, acc);
+            backward_point(&q, p, grad_shading, grad_diffuse_shading, gn, gv, ga, gs, gr, gr_abs,
+                           acc);
     }
     if (gE) {
         for (int i = 0; i < nt; i++)
@@ -508,4 +522,19 @@ int des_oracle_backward(int variant, int enable_diffuse, int enable_specular, in
     free(tl);
     free(tdw);
     return 0;
+}
+
+/* The backward without the conditioning scale of grad_roughness. */
+int des_oracle_backward(int variant, int enable_diffuse, int enable_specular, int64_t P, int H,
+                        int W, int N_face, int N_vertex, const double* grad_shading,
+                        const double* grad_diffuse_shading, const double* normal,
+                        const double* view_dir, const double* albedo, const double* spec_albedo,
+                        const double* rough, const double* envmap, const double* rotmat,
+                        const int32_t* fvl, const int64_t* tri, const double* bary,
+                        const uint32_t* vis, double* gn, double* gv, double* ga, double* gs,
+                        double* gr, double* gE, int nthreads) {
+    return des_oracle_backward_ex(variant, enable_diffuse, enable_specular, P, H, W, N_face,
+                                  N_vertex, grad_shading, grad_diffuse_shading, normal, view_dir,
+                                  albedo, spec_albedo, rough, envmap, rotmat, fvl, tri, bary, vis,
+                                  gn, gv, ga, gs, gr, gE, NULL, nthreads);
 }

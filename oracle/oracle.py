@@ -54,6 +54,8 @@ def lib():
         L.des_oracle_backward.argtypes = [c_int, c_int, c_int, c_i64, c_int, c_int, c_int, c_int,
                                           dp, dp, dp, dp, dp, dp, dp, dp, dp, i32p, i64p, dp,
                                           u32p, dp, dp, dp, dp, dp, dp, c_int]
+        L.des_oracle_backward_ex.restype = c_int
+        L.des_oracle_backward_ex.argtypes = L.des_oracle_backward.argtypes[:-1] + [dp, c_int]
         L.des_oracle_texel_table.restype = None
         L.des_oracle_texel_table.argtypes = [c_int, c_int, dp, dp, dp]
         L.des_oracle_delta_omega.restype = ctypes.c_double
@@ -165,9 +167,13 @@ def forward(variant: str, inp: dict, enable_diffuse: bool = True, enable_specula
 
 
 def backward(variant: str, inp: dict, grad_shading, grad_diffuse_shading=None,
-             enable_diffuse: bool = True, enable_specular: bool = True, nthreads: int = 0):
+             enable_diffuse: bool = True, enable_specular: bool = True, nthreads: int = 0,
+             roughness_abs: bool = False):
     """Returns dict of float64 grads: normal, view_dir, albedo, specular_albedo,
-    roughness (P x 1) and envmap (3 x H x W)."""
+    roughness (P x 1) and envmap (3 x H x W); with roughness_abs also
+    "roughness_abs" (P x 1): the magnitudes of the terms summed into the
+    alpha^2 and k gradients (two each per texel), through the chain factors
+    of the roughness gradient -- the scale of that sum's rounding error."""
     v = VARIANTS[variant]
     P, H, W, NF, NV, ptrs, keep = _common(inp)
     gsh, gshp = _d(grad_shading)
@@ -176,10 +182,13 @@ def backward(variant: str, inp: dict, grad_shading, grad_diffuse_shading=None,
     out["roughness"] = np.zeros((P, 1))
     out["envmap"] = np.zeros((3, H, W))
     dp = ctypes.POINTER(ctypes.c_double)
-    rc = lib().des_oracle_backward(
+    if roughness_abs:
+        out["roughness_abs"] = np.zeros((P, 1))
+    rc = lib().des_oracle_backward_ex(
         v, int(enable_diffuse), int(enable_specular), P, H, W, NF, NV, gshp, gdsp, *ptrs,
         *(out[k].ctypes.data_as(dp) for k in ("normal", "view_dir", "albedo", "specular_albedo",
-                                              "roughness", "envmap")), nthreads)
+                                              "roughness", "envmap")),
+        out["roughness_abs"].ctypes.data_as(dp) if roughness_abs else None, nthreads)
     if rc != 0:
         raise RuntimeError(f"oracle backward failed ({rc})")
     return out

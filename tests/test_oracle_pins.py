@@ -316,3 +316,41 @@ def test_back_facing_views_shade_specular_except_in_the_clamp_variant():
         assert (sp[0] > 0).all()
         cl, _ = O.forward("clamp", d, enable_diffuse=False, enable_specular=True)
         assert (cl[0] == 0).all()
+
+
+# ---- the conditioning scale of grad_roughness --------------------------------------
+
+def test_roughness_scale_bounds_the_one_hot_terms():
+    # grad_roughness is linear in the env map, so with every texel but t set
+    # to zero it is exactly texel t's term.  The scale the oracle reports sums
+    # the magnitudes of the parts of every texel's term (each texel's parts
+    # depend on its own radiance only), so it is additive over one-hot maps
+    # and bounds the sum of the terms' magnitudes, and asking for it changes
+    # no output
+    d = _d64(problem(seed=8, n=60, H=4, W=8, vis="identity"))
+    gs, gd = d["grad_shading"], d["grad_diffuse_shading"]
+    g = O.backward("vis", d, gs, gd, roughness_abs=True)
+    g0 = O.backward("vis", d, gs, gd)
+    assert "roughness_abs" not in g0
+    for k in g0:
+        assert np.array_equal(g[k], g0[k]), k
+    T = 32
+    terms = np.zeros((60, T))
+    scales = np.zeros((60, T))
+    for t in range(T):
+        e = np.zeros_like(d["envmap"])
+        e[:, t // 8, t % 8] = d["envmap"][:, t // 8, t % 8]
+        gt = O.backward("vis", dict(d, envmap=e), gs, gd, roughness_abs=True)
+        terms[:, t] = gt["roughness"][:, 0]
+        scales[:, t] = gt["roughness_abs"][:, 0]
+    np.testing.assert_allclose(terms.sum(1), g["roughness"][:, 0], rtol=1e-10, atol=1e-14)
+    np.testing.assert_allclose(scales.sum(1), g["roughness_abs"][:, 0], rtol=1e-12, atol=0)
+    assert np.all(scales >= np.abs(terms) * (1 - 1e-12))
+    # the terms cancel across texels and their parts within texels
+    assert np.any(np.abs(terms).sum(1) > 2 * np.abs(g["roughness"][:, 0]))
+    assert np.any(scales > 2 * np.abs(terms))
+    # scaling the radiance scales it; no specular lobe: no roughness terms
+    g3 = O.backward("vis", dict(d, envmap=3.0 * d["envmap"]), gs, gd, roughness_abs=True)
+    np.testing.assert_allclose(g3["roughness_abs"], 3.0 * g["roughness_abs"], rtol=1e-12)
+    gn = O.backward("vis", d, gs, gd, enable_specular=False, roughness_abs=True)
+    assert np.all(gn["roughness_abs"] == 0)
