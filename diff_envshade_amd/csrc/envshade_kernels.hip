@@ -90,6 +90,24 @@ __device__ void build_table(const Problem& pr, int t0, int n, int n_alloc, Texel
   }
 }
 
+// The forward's table (FwdTexel entries), same texels and zero padding.
+__device__ void build_fwd_table(const Problem& pr, int t0, int n, int n_alloc, FwdTexel* tab) {
+  double M[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) M[i] = (double)pr.rotmat[i];
+  const double kdw = solid_angle_scale(pr.H, pr.W);
+  const int T = pr.T;
+  for (int i = threadIdx.x; i < n_alloc; i += blockDim.x) {
+    if (i < n) {
+      const int t = t0 + i;
+      tab[i] = make_fwd_texel(texel_geometry(t, pr.H, pr.W, M, kdw), pr.envmap[t],
+                              pr.envmap[T + t], pr.envmap[2 * T + t]);
+    } else {
+      tab[i] = zero_fwd_texel();
+    }
+  }
+}
+
 // ---- per-point state ---------------------------------------------------------
 
 template <int V, bool SPEC>
@@ -255,14 +273,16 @@ __device__ __forceinline__ Cone tile_cone(const PointState<V, SPEC>& s) {
 
 // Dark bits of texels t0 .. t0+63 of the staged super-chunk (sn texels), bit
 // i for texel t0 + i; texels past sn read as dark.  Wave-uniform.
-__device__ __forceinline__ uint64_t dark_word(const Texel* tab, const Cone& c, int t0, int sn,
+template <class TX>
+__device__ __forceinline__ uint64_t dark_word(const TX* tab, const Cone& c, int t0, int sn,
                                               int lane) {
   const int t = t0 + lane;
   bool dark = true;
   if (t < sn) {
-    const Texel& g = tab[t];
-    const float len = __builtin_amdgcn_sqrtf(fmaf(g.lz, g.lz, fmaf(g.ly, g.ly, g.lx * g.lx)));
-    dark = fmaf(g.lz, c.az, fmaf(g.ly, c.ay, g.lx * c.ax)) <= -c.sb * len;
+    float lx, ly, lz;
+    texel_dir(tab[t], lx, ly, lz);
+    const float len = __builtin_amdgcn_sqrtf(fmaf(lz, lz, fmaf(ly, ly, lx * lx)));
+    dark = fmaf(lz, c.az, fmaf(ly, c.ay, lx * c.ax)) <= -c.sb * len;
   }
   return __ballot(dark);
 }
@@ -276,8 +296,8 @@ __device__ __forceinline__ uint64_t dark_word(const Texel* tab, const Cone& c, i
 // depends only on the tile's dark bits, so the sums are reproducible.
 // lit(tg, d4) / dark(tg) get the group's first texel and, for lit groups, its
 // 4 dark bits.
-template <class Lit, class Dark>
-__device__ __forceinline__ void for_groups(const Texel* tab, const Cone& cone, int sn, int wv,
+template <class TX, class Lit, class Dark>
+__device__ __forceinline__ void for_groups(const TX* tab, const Cone& cone, int sn, int wv,
                                            int ntc, int lane, Lit&& lit, Dark&& dark) {
   const int ngt = (sn + 3) >> 2;
   // a cone wider than a hemisphere darkens no texel: skip the dark bits
@@ -318,30 +338,40 @@ __device__ __forceinline__ void for_groups(const Texel* tab, const Cone& cone, i
 
 // One wave's share of the texel groups of one staged super-chunk (sn texels).
 // A group that lights some lane is shaded whole, without a branch per texel,
-// so the four lobes form one block the compiler can interleave; its dark
+// so the four pairs form one block the compiler can interleave; its dark
 // texels add exact zeros (see Cone).  SINGLE: the wave-uniform case of one
-// visibility word per slot, chosen once per super-chunk.
+// visibility word per slot, chosen once per super-chunk; the word's bit is
+// applied as a mask and the lane's weight (the sum of its barycentrics) by
+// the caller, once per sum.  Other variants' per-lane weight (live) is also
+// applied by the caller.
 // acc = {sum E vis max(l.n,0) per channel, sum E vis F G D / 4 per channel}.
 template <int V, bool DIFF, bool SPEC, bool SINGLE>
-__device__ __forceinline__ void forward_chunk(const PointState<V, SPEC>& st, const Texel* tab,
+__device__ __forceinline__ void forward_chunk(const PointState<V, SPEC>& st, const FwdTexel* tab,
                                               const uint32_t* svis, const Cone& cone, int sn,
-                                              int wv, int ntc, int lane, float csum,
-                                              float (&acc)[6]) {
+                                              int wv, int ntc, int lane, float (&acc)[6]) {
+  constexpr bool MASK = (V != kVis) || SINGLE;
   for_groups(tab, cone, sn, wv, ntc, lane, [&](int tg, uint32_t) {
     const VisWords w = staged_words<V>(svis, tg >> 5, SINGLE, lane);
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      const Texel& g = tab[tg + i];
-      const float vis = vis_weight<V, SPEC>(st, w, (tg & 31) + i, SINGLE, csum);
-      const Lobe lb = eval_lobe<SPEC>(st, g);
+      const FwdTexel& g = tab[tg + i];
+      const int b = (tg & 31) + i;
+      uint32_t mask = ~0u;
+      float vis = 1.f;
+      if (V == kVis) {
+        if (SINGLE)
+          mask = (uint32_t)__builtin_amdgcn_sbfe((int)w.w0, b, 1);  // 0 or ~0
+        else
+          vis = vis_weight<V, SPEC>(st, w, b, false, 0.f);
+      }
+      float x, y;
+      fwd_pair<SPEC, MASK>(st, g, mask, vis, x, y);
       if (DIFF) {
-        const float x = lb.nl * vis;
         acc[0] = fmaf(g.e0, x, acc[0]);
         acc[1] = fmaf(g.e1, x, acc[1]);
         acc[2] = fmaf(g.e2, x, acc[2]);
       }
       if (SPEC) {
-        const float y = lb.tmp * vis;
         acc[3] = fmaf(g.e0, y, acc[3]);
         acc[4] = fmaf(g.e1, y, acc[4]);
         acc[5] = fmaf(g.e2, y, acc[5]);
@@ -406,19 +436,24 @@ __device__ __forceinline__ void forward_tile(const Problem& pr, const Tiling& tl
   constexpr int NACC = kForwardAcc;
   float acc[NACC] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
 
-  Texel* tab = reinterpret_cast<Texel*>(smem);
+  FwdTexel* tab = reinterpret_cast<FwdTexel*>(smem);
   uint32_t* svis = reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(smem) + tl.vis_off);
   for (int s0 = t_begin; s0 < t_end; s0 += tl.st) {
     const int sn = min(tl.st, t_end - s0);
     __syncthreads();
-    build_table(pr, s0, sn, min(tl.st, (sn + 31) & ~31), tab);
+    build_fwd_table(pr, s0, sn, min(tl.st, (sn + 31) & ~31), tab);
     stage_vis(pr, st, s0, sn, single, svis, wv, tl.ntc, lane);
     __syncthreads();
     if (single)
-      forward_chunk<V, DIFF, SPEC, true>(st, tab, svis, cone, sn, wv, tl.ntc, lane, csum, acc);
+      forward_chunk<V, DIFF, SPEC, true>(st, tab, svis, cone, sn, wv, tl.ntc, lane, acc);
     else
-      forward_chunk<V, DIFF, SPEC, false>(st, tab, svis, cone, sn, wv, tl.ntc, lane, csum, acc);
+      forward_chunk<V, DIFF, SPEC, false>(st, tab, svis, cone, sn, wv, tl.ntc, lane, acc);
   }
+  // the per-lane weight the masked sums leave out: the barycentric sum of a
+  // single-word visibility tile, or the live flag of the other variants
+  const float wl = (V == kVis) ? (single ? csum : 1.f) : st.live;
+#pragma unroll
+  for (int j = 0; j < NACC; ++j) acc[j] *= wl;
 
   // fixed-order reduction over the NTC texel chunks
   if (tl.ntc > 1) {

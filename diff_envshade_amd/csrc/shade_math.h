@@ -83,6 +83,44 @@ __device__ __forceinline__ Texel zero_texel() {
   return o;
 }
 
+// One texel as the forward reads it (48 bytes): l = M d and |l|^2 in FP64,
+// E_c * dOmega in fp32.  The forward forms every dot product of its lobe in
+// FP64 from these, so it needs no fp32 copy of l.
+struct FwdTexel {
+  double dx, dy, dz;
+  double l2;
+  float e0, e1, e2;
+  float pad;
+};
+static_assert(sizeof(FwdTexel) == sizeof(Texel), "forward texel entry layout");
+
+__device__ __forceinline__ FwdTexel make_fwd_texel(const TexelGeo& g, float E0, float E1,
+                                                   float E2) {
+  FwdTexel o;
+  o.dx = g.lx; o.dy = g.ly; o.dz = g.lz;
+  o.l2 = fma(g.lz, g.lz, fma(g.ly, g.ly, g.lx * g.lx));
+  o.e0 = (float)((double)E0 * g.dw);
+  o.e1 = (float)((double)E1 * g.dw);
+  o.e2 = (float)((double)E2 * g.dw);
+  o.pad = 0.f;
+  return o;
+}
+
+__device__ __forceinline__ FwdTexel zero_fwd_texel() {
+  FwdTexel o;
+  o.dx = o.dy = o.dz = o.l2 = 0.0;
+  o.e0 = o.e1 = o.e2 = o.pad = 0.f;
+  return o;
+}
+
+// The light direction of a table entry rounded to fp32 (the cone test's input).
+__device__ __forceinline__ void texel_dir(const Texel& g, float& x, float& y, float& z) {
+  x = g.lx; y = g.ly; z = g.lz;
+}
+__device__ __forceinline__ void texel_dir(const FwdTexel& g, float& x, float& y, float& z) {
+  x = (float)g.dx; y = (float)g.dy; z = (float)g.dz;
+}
+
 // ---- the per-pair lobe -------------------------------------------------------
 
 // Quantities of one (point, texel) pair shared by forward and backward.
@@ -178,6 +216,8 @@ struct PointGeom {
   double nxd, nyd, nzd, vxd, vyd, vzd;
   float nv, k, omk, ib4, a2, a2pi, c4;
   float n2c;  // 1 - |n|^2 (formed in FP64), for eval_weights_f32
+  double nvd, v2d;  // n.v and |v|^2 in FP64, for fwd_pair
+  float cD;         // ib4 * a2pi: the per-point factor of F G D / 4, for fwd_pair
 };
 
 __device__ __forceinline__ PointGeom point_geom(float nx, float ny, float nz, float vx, float vy,
@@ -197,7 +237,52 @@ __device__ __forceinline__ PointGeom point_geom(float nx, float ny, float nz, fl
   c.omk = 1.f - c.k;
   c.ib4 = 0.25f / (fmaxf(c.nv, 0.f) * c.omk + c.k);
   c.n2c = (float)(1.0 - (c.nxd * c.nxd + c.nyd * c.nyd + c.nzd * c.nzd));
+  c.nvd = fma(c.nzd, c.vzd, fma(c.nyd, c.vyd, c.nxd * c.vxd));
+  c.v2d = fma(c.vzd, c.vzd, fma(c.vyd, c.vyd, c.vxd * c.vxd));
+  c.cD = c.ib4 * c.a2pi;
   return c;
+}
+
+// The forward's two weights of one pair, each times the pair's visibility:
+//   x = vis max(l.n, 0),   y = vis F G D / 4  (0 where the reference skips the lobe).
+// MASK: vis is 0 or 1, given as mask = 0 or ~0 and applied to max(l.n, 0) as a
+// bit mask; otherwise vis is the float weight vis_f.  G = max(l.n,0) ia ib is
+// linear in max(l.n,0) (ia from the unmasked value), so y carries the weight
+// through x.
+//
+// Every dot product is formed in FP64 from the table's FP64 direction l:
+//   l.n, l.v;  l.hb = |l|^2 + l.v;  |hb|^2 = l.hb + (l.v + |v|^2);  n.hb = l.n + n.v,
+// so the NDF denominator
+//   tD = (n.h)^2 alpha^2 + (1 - (n.h)^2) = ((n.hb)^2 alpha^2 + (|hb|^2 - (n.hb)^2)) / |hb|^2
+// takes |hb|^2 - (n.hb)^2 from FP64 (no cancellation at the peak of sharp
+// lobes), and l.h = l.hb / |hb| gives the Fresnel term.  The two
+// reciprocals of G1(l) and D share one: F G D / 4 = F x (ib/4) (alpha^2/pi) /
+// (max(tD^2, thres/pi) (max(l.n,0)(1-k) + k)).  Same terms as eval_lobe.
+template <bool SPEC, bool MASK>
+__device__ __forceinline__ void fwd_pair(const PointGeom& s, const FwdTexel& g, uint32_t mask,
+                                         float vis_f, float& x, float& y) {
+  const double nl = fma(g.dz, s.nzd, fma(g.dy, s.nyd, g.dx * s.nxd));
+  const float nlf = fmaxf((float)nl, 0.f);
+  x = MASK ? __uint_as_float(__float_as_uint(nlf) & mask) : nlf * vis_f;
+  y = 0.f;
+  if (!SPEC) return;
+  const double lv = fma(g.dz, s.vzd, fma(g.dy, s.vyd, g.dx * s.vxd));
+  const double lhb = lv + g.l2;
+  const double hb2 = lhb + (lv + s.v2d);
+  const double nhb = nl + s.nvd;
+  const float sn2 = (float)fma(-nhb, nhb, hb2);  // |hb|^2 (1 - (n.h)^2)
+  const float hb2f = (float)hb2;
+  const float nhbf = (float)nhb;
+  // floor keeps the terms finite when v = -l exactly; such pairs are masked below
+  const float rs = rsq(fmaxf(hb2f, 1e-20f));
+  const float om = 1.f - sat((float)lhb * rs);
+  const float om2 = om * om;
+  const float F = fmaf(om2 * om2 * om, 1.f - kF0, kF0);
+  const float tD = fmaf(nhbf * nhbf, s.a2, sn2) * (rs * rs);
+  const float den = fmaxf(tD * tD, kThresOverPi) * fmaf(nlf, s.omk, s.k);
+  const float tmp = ((F * x) * s.cD) * rcp(den);
+  // the reference skips the lobe unless tmp > 0; |l+v| = 0 gives NaN there
+  y = (tmp > 0.f && hb2 > 0.0) ? tmp : 0.f;
 }
 
 // The two weights of a pair in fp32 only, for consumers that round them to
