@@ -98,7 +98,6 @@ des::Tiling tiling_for(int ntc, int T, bool backward) {
   tl.vis_off = int(table);
   tl.lds_fwd = std::max(staged, size_t(ntc) * des::kForwardAcc * 64 * sizeof(float));
   tl.lds_bwd = std::max(staged, size_t(ntc) * des::kBackwardAcc * 64 * sizeof(float));
-  tl.reduce_rows = des::kReduceRows;
   tl.split0 = INT64_MAX;  // set per problem by choose_tiling (single-set launches only)
   tl.slices = 1;
   tl.slice_texels = T;
@@ -173,21 +172,19 @@ bool use_order(int64_t P, int T, bool backward) {
 }
 
 struct WsLayout {
-  size_t part_off, part2_off, spart_off, order_off, total;
+  size_t part_off, spart_off, order_off, total;
   bool ordered;
 };
 
-// Backward workspace: per-tile fp32 partials, fp64 group sums, the per-point
-// sums of the tail slices (single-set launches), and (when the points are
-// reordered) the point-order workspace.
+// Backward workspace: per-tile fp32 partials, the per-point sums of the tail
+// slices (single-set launches), and (when the points are reordered) the
+// point-order workspace.
 WsLayout ws_layout(int64_t P, int T, bool allow_order) {
-  WsLayout w{0, 0, 0, 0, 0, false};
+  WsLayout w{0, 0, 0, 0, false};
   if (P <= 0) return w;
   const int64_t tiles = n_tiles(P);
-  const int64_t groups = (tiles + des::kReduceRows - 1) / des::kReduceRows;
   w.part_off = 0;
-  w.part2_off = align256(size_t(tiles) * 3 * T * sizeof(float));
-  w.total = w.part2_off + align256(size_t(groups) * 3 * T * sizeof(double));
+  w.total = align256(size_t(tiles) * 3 * T * sizeof(float));
   if (allow_order) {
     const TailSplit ts = tail_split(P, T);
     w.spart_off = w.total;
@@ -304,7 +301,6 @@ int run_backward(int variant, bool diff, bool spec, des::Problem& pr, void* ws, 
   if ((reinterpret_cast<uintptr_t>(ws) & 255) != 0)
     return fail(DES_ERR_INVALID_ARGUMENT, "workspace must be 256-byte aligned");
   pr.part = reinterpret_cast<float*>(static_cast<char*>(ws) + lay.part_off);
-  pr.part2 = reinterpret_cast<double*>(static_cast<char*>(ws) + lay.part2_off);
   pr.spart = reinterpret_cast<float*>(static_cast<char*>(ws) + lay.spart_off);
   if (lay.ordered && reuse != nullptr) {
     pr.order = reinterpret_cast<const int32_t*>(static_cast<const char*>(reuse) +
@@ -356,7 +352,7 @@ static_assert(sizeof(des::BatchHeader) <= kHeaderBytes, "header too large");
 struct BatchLayout {
   size_t sets_off, jobs_off, table_bytes, total;
   int njobs;
-  std::vector<size_t> part_off, part2_off, order_off;  // order_off: 0 = not sorted
+  std::vector<size_t> part_off, order_off;  // order_off: 0 = not sorted
 };
 
 BatchLayout batch_layout(const des_set* sets, int n) {
@@ -370,12 +366,10 @@ BatchLayout batch_layout(const des_set* sets, int n) {
                            sizeof(des::SortJob) * size_t(L.njobs));
   size_t off = L.table_bytes;
   L.part_off.resize(n);
-  L.part2_off.resize(n);
   L.order_off.assign(n, 0);
   for (int i = 0; i < n; ++i) {
     const WsLayout w = ws_layout(sets[i].P, sets[i].H * sets[i].W, false);
     L.part_off[i] = off + w.part_off;
-    L.part2_off[i] = off + w.part2_off;
     off += w.total;
     if (use_order(sets[i].P, sets[i].H * sets[i].W, true)) {
       L.order_off[i] = off;
@@ -450,14 +444,14 @@ des::BatchLaunch batch_launch(void* ws, int variant, bool diff, bool spec, bool 
   bl.lds = fwd ? tl.lds_fwd : tl.lds_bwd;
   const int per_cu = des::batch_blocks_per_cu(variant, diff, spec, fwd, ntc, bl.lds);
   bl.grid = unsigned(num_cus() * per_cu);
-  bl.reduce_grid = unsigned(num_cus() * 4);
+  bl.reduce_grid = unsigned(num_cus());
   return bl;
 }
 
 // ---- scratch of the entry point that allocates nothing -------------------------
 //
 // des_bruteforce_specularvisibility_shader_backward2_noalloc keeps its
-// workspace (per-tile env-map-gradient partials, FP64 group sums, point order)
+// workspace (per-tile env-map-gradient partials, tail-slice sums, point order)
 // in this array instead of a caller's buffer.  Each GPU holds its own copy
 // from the library's load to the end of the process; in host code the name
 // is only a stand-in, so the device address of each GPU's copy comes from
@@ -877,7 +871,7 @@ DES_API int des_batch_prepare(int32_t variant, int enable_diffuse, int enable_sp
   auto* tab = reinterpret_cast<des::SortJobs*>(st + L.jobs_off);
   des::SortJob* jobs = tab->jobs;
   int njobs = 0, sort_blocks = 0, sort_scan = 0;
-  int64_t tile0 = 0, r1 = 0, r2 = 0;
+  int64_t tile0 = 0, red = 0;
   for (int i = 0; i < n_sets; ++i) {
     const des_set& s = sets[i];
     des::Problem pr = make_problem(s.P, s.H, s.W);
@@ -892,7 +886,6 @@ DES_API int des_batch_prepare(int32_t variant, int enable_diffuse, int enable_sp
     pr.gn = s.grad_normal; pr.gv = s.grad_view_dir; pr.ga = s.grad_albedo;
     pr.gs = s.grad_specular_albedo; pr.gr = s.grad_roughness; pr.gE = s.grad_envmap;
     pr.part = reinterpret_cast<float*>(ws + L.part_off[i]);
-    pr.part2 = reinterpret_cast<double*>(ws + L.part2_off[i]);
     des::BatchSet& b = bs[i];
     b.pr = pr;
     if (L.order_off[i] != 0) {
@@ -909,14 +902,9 @@ DES_API int des_batch_prepare(int32_t variant, int enable_diffuse, int enable_sp
     b.tl_bwd = tiling_for(ntc_backward, pr.T, true);
     const int64_t tiles = n_tiles(s.P);
     b.tile0 = tile0;
-    b.ngroups = int32_t((tiles + des::kReduceRows - 1) / des::kReduceRows);
-    b.colblk = (3 * pr.T + 255) / 256;
-    b.colblk16 = (3 * pr.T + 15) / 16;
-    b.r1_item0 = r1;
-    b.r2_item0 = r2;
+    b.red_item0 = red;
     tile0 += tiles;
-    r1 += int64_t(b.ngroups) * b.colblk;
-    r2 += b.colblk16;
+    red += (3 * pr.T + 15) / 16;
   }
   hdr->n_sets = n_sets;
   hdr->variant = variant;
@@ -924,8 +912,7 @@ DES_API int des_batch_prepare(int32_t variant, int enable_diffuse, int enable_sp
   hdr->sig_fwd = des::batch_signature(variant, diff, spec, ntc_forward);
   hdr->sig_bwd = des::batch_signature(variant, diff, spec, ntc_backward);
   hdr->total_tiles = tile0;
-  hdr->total_r1 = r1;
-  hdr->total_r2 = r2;
+  hdr->total_red = red;
   hdr->sort_jobs = reinterpret_cast<const des::SortJobs*>(ws + L.jobs_off);
   tab->njobs = njobs;
   tab->nblocks = sort_blocks;

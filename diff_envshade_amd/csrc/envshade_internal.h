@@ -43,7 +43,6 @@ struct Problem {
   float* gE;              // 3 x H x W
   const int32_t* order;   // P: point of each tile lane (tile*64 + lane), or null = identity
   float* part;            // [ceil(P/64)][3][T] per-tile env-map partials
-  double* part2;          // [ngroups][3][T] stage-1 FP64 sums
   float* spart;           // backward tail slices: [slice item][kBackwardAcc][64] per-point sums
 };
 
@@ -55,10 +54,9 @@ struct Tiling {
   int vis_off;      // byte offset of the staged visibility words in dynamic LDS
   size_t lds_fwd;   // dynamic LDS bytes, forward
   size_t lds_bwd;   // dynamic LDS bytes, backward
-  int reduce_rows;  // partial rows summed per stage-1 group
   // Backward tail slices: tiles [split0, tiles) are shaded as `slices`
   // workgroups each, slice s covering texels [s * slice_texels, ...), after the
-  // whole tiles; their per-point sums are combined by the stage-1 reduce.
+  // whole tiles; their per-point sums are combined by the env-gradient reduce launch.
   // split0 = tiles and slices = 1 when nothing is sliced.
   int64_t split0;
   int slices;
@@ -69,17 +67,13 @@ struct SortJobs;  // the point-order jobs of a batch (below)
 
 // Batched launches: a device-resident descriptor table.
 struct BatchSet {
-  Problem pr;         // pointers + sizes of one point set (part/part2 inside the workspace)
+  Problem pr;         // pointers + sizes of one point set (part inside the workspace)
   const int32_t* order_fwd;  // point order the forward shades in, or null (input order)
   const int32_t* order_bwd;  // the same for the backward
   Tiling tl_fwd;      // ntc equals the forward launch's ntc; cw/st follow this set's T
   Tiling tl_bwd;      // same for the backward launch
   int64_t tile0;      // first global tile of this set
-  int64_t r1_item0;   // first stage-1 reduce item (ngroups * colblk items per set)
-  int64_t r2_item0;   // first stage-2 reduce item (colblk items per set)
-  int32_t ngroups;    // ceil(tiles / reduce_rows)
-  int32_t colblk;     // ceil(3 T / 256) (stage 1)
-  int32_t colblk16;   // ceil(3 T / 16) (stage 2)
+  int64_t red_item0;  // first env-gradient reduce item (one per 16 columns of 3 T)
 };
 
 // Launch signature of a prepared batch: variant, branch flags and the waves per
@@ -98,8 +92,7 @@ struct BatchHeader {
   int32_t sig_fwd;       // batch_signature(variant, diff, spec, ntc_forward) at prepare
   int32_t sig_bwd;       // the same with ntc_backward
   int64_t total_tiles;
-  int64_t total_r1;
-  int64_t total_r2;
+  int64_t total_red;  // env-gradient reduce items of all sets
   unsigned long long fwd_counter;  // work counters, zeroed by a kernel at each launch
   unsigned long long bwd_counter;
   const SortJobs* sort_jobs;       // the sets' point-order jobs (inside the table)
@@ -126,14 +119,13 @@ constexpr int kForwardAcc = 6;
 constexpr int kBackwardAcc = 16;
 constexpr int kMaxSuperChunk = 512;  // texels staged in LDS at once (48 B each, 24 KB)
 constexpr int kTexelBytes = 48;
-constexpr int kReduceRows = 16;
 
 hipError_t launch_forward(int variant, bool diff, bool spec, const Problem& pr, const Tiling& tl,
                           hipStream_t stream);
 hipError_t launch_backward(int variant, bool diff, bool spec, const Problem& pr, const Tiling& tl,
                            hipStream_t stream);
-// Stage 1 (and the combine of the backward's tail slices) and stage 2 of the
-// env-map-gradient reduction.
+// The env-map-gradient reduction and the combine of the backward's tail
+// slices, one launch.
 hipError_t launch_envgrad_reduce(int variant, bool diff, bool spec, const Problem& pr,
                                  const Tiling& tl, hipStream_t stream);
 hipError_t launch_fill_zero(float* x, int64_t n, hipStream_t stream);

@@ -753,14 +753,12 @@ __global__ __launch_bounds__(1024) void shade_forward_kernel(Problem pr, Tiling 
 }
 
 // The forward's tail slices: per point the slices' sums in slice order, then
-// the point's outputs.
+// the point's outputs.  One tail tile per 64-thread block, so the scattered
+// loads and stores spread over as many CUs as there are tail tiles.
 template <int V, bool DIFF, bool SPEC>
-__global__ __launch_bounds__(256) void forward_combine_kernel(Problem pr, Tiling tl) {
-  const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  const int64_t ntail = (pr.P + 63) / 64 - tl.split0;  // >= 0 (checked at launch)
-  if (idx >= ntail * 64) return;
-  const int64_t k = idx >> 6;
-  const int lane = (int)(idx & 63);
+__global__ __launch_bounds__(64) void forward_combine_kernel(Problem pr, Tiling tl) {
+  const int64_t k = blockIdx.x;  // tail tile k: tile split0 + k (< tiles, checked at launch)
+  const int lane = (int)threadIdx.x;
   const int64_t q = (tl.split0 + k) * 64 + lane;
   if (q >= pr.P) return;
   const int64_t p = pr.order != nullptr ? (int64_t)pr.order[q] : q;
@@ -855,80 +853,74 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(3))) void b
 
 // ---- env-map gradient: fixed-order FP64 reduction of the per-tile partials -----
 
-// stage 1: rows [g*R, min((g+1)*R, nrows)) of part[nrows][ncol] -> part2[g][ncol],
-// columns [cb*256, cb*256+256)
-__device__ __forceinline__ void reduce_rows_item(const float* __restrict__ part,
-                                                 double* __restrict__ part2, int64_t nrows,
-                                                 int ncol, int R, int64_t g, int cb) {
-  const int col = cb * 256 + threadIdx.x;
-  if (col >= ncol) return;
-  const int64_t r0 = g * R;
-  const int64_t r1 = min<int64_t>(r0 + R, nrows);
-  double s = 0.0;
-  int64_t r = r0;
-  for (; r + 4 <= r1; r += 4) {
-    const float a = part[r * ncol + col];
-    const float b = part[(r + 1) * ncol + col];
-    const float c = part[(r + 2) * ncol + col];
-    const float d = part[(r + 3) * ncol + col];
-    s += (double)a;
-    s += (double)b;
-    s += (double)c;
-    s += (double)d;
-  }
-  for (; r < r1; ++r) s += (double)part[r * ncol + col];
-  part2[g * ncol + col] = s;
-}
+// gE[c][t] = dOmega_t * sum over the tile rows of part[row][c*T + t], in FP64,
+// rounded once to fp32.  One 1024-thread block owns 16 columns: lane l of
+// wave w reads column (l & 15) of the rows r0_w + (l >> 4) + 4 i, where wave w
+// covers rows [w*nrows/16, (w+1)*nrows/16) -- four 64-byte row segments per
+// load, every load of a lane issued before the first add.  The four partial
+// sums of a column in a wave are added in sub-row order, then wave 0 adds the
+// 16 wave sums in order, so the FP64 sum order depends only on nrows.
+constexpr int kReduceWaves = 16;
+constexpr int kReduceCols = 16;
 
-// stage 2: gE[c][t] = dOmega_t * sum_g part2[g][c*T + t], rounded once to fp32.
-// A 256-thread block owns 16 columns; thread (column j, lane q) sums groups
-// q, q+16, ... and a fixed-order LDS tree folds the 16 lanes, so the FP64 sum
-// order depends only on ngroups.
-__device__ __forceinline__ void finish_item(const double* __restrict__ part2,
-                                            float* __restrict__ gE, int ngroups, int H, int W,
-                                            int cb16) {
-  __shared__ double s_sum[16][17];
+__device__ __forceinline__ void reduce_cols_item(const float* __restrict__ part,
+                                                 float* __restrict__ gE, int64_t nrows, int H,
+                                                 int W, int cb) {
+  __shared__ double s_sum[kReduceWaves][kReduceCols];
   const int T = H * W;
-  const int j = threadIdx.x & 15;
-  const int q = threadIdx.x >> 4;
-  const int col = cb16 * 16 + j;
+  const int ncol = 3 * T;
+  const int lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
+  const int j = lane & 15;
+  const int rs = lane >> 4;
+  const int col = cb * kReduceCols + j;
+  const int64_t r0 = nrows * w / kReduceWaves;
+  const int64_t r1 = nrows * (w + 1) / kReduceWaves;
   double s = 0.0;
-  if (col < 3 * T)
-    for (int g = q; g < ngroups; g += 16) s += part2[(int64_t)g * 3 * T + col];
-  s_sum[q][j] = s;
-  __syncthreads();
+  if (col < ncol) {
+    const float* src = part + col;
+    int64_t r = r0 + rs;
+    for (; r + 28 < r1; r += 32) {
+      float v[8];
 #pragma unroll
-  for (int step = 8; step > 0; step >>= 1) {
-    if (q < step) s_sum[q][j] += s_sum[q + step][j];
-    __syncthreads();
+      for (int i = 0; i < 8; ++i) v[i] = src[(r + 4 * i) * ncol];
+#pragma unroll
+      for (int i = 0; i < 8; ++i) s += (double)v[i];
+    }
+    for (; r < r1; r += 4) s += (double)src[r * ncol];
   }
-  if (q == 0 && col < 3 * T) {
-    const int t = col % T;
-    const int iy = H - 1 - t / W;
-    const double dw = solid_angle_scale(H, W) * sinpi(((double)iy + 0.5) / (double)H);
-    gE[col] = (float)(s_sum[0][j] * dw);
-  }
+  // sub-rows 0..3 of each column, in order: lanes j, j+16, j+32, j+48
+  const double s1 = __shfl(s, j + 16, 64), s2 = __shfl(s, j + 32, 64), s3 = __shfl(s, j + 48, 64);
+  if (rs == 0) s_sum[w][j] = ((s + s1) + s2) + s3;
   __syncthreads();
+  if (w == 0 && rs == 0 && col < ncol) {
+    double t = 0.0;
+#pragma unroll
+    for (int k = 0; k < kReduceWaves; ++k) t += s_sum[k][j];
+    const int tt = col % T;
+    const int iy = H - 1 - tt / W;
+    const double dw = solid_angle_scale(H, W) * sinpi(((double)iy + 0.5) / (double)H);
+    gE[col] = (float)(t * dw);
+  }
+  __syncthreads();  // s_sum is reused by the next item of a persistent block
 }
 
-// Stage 1, and in the first ncomb rows (blockIdx.x == 0) the combine of the
-// backward's tail slices: per point the slices' sums in slice order, then the
-// point's gradients.  The combine rows come first so that their chain of
-// dependent loads overlaps the reduction rows instead of trailing them.
+// One launch: the first ncomb blocks combine the backward's tail slices, one
+// tile per block in its first wave (spread over as many CUs as there are tail
+// tiles, since the combine is a chain of scattered loads and stores): per
+// point the slices' sums in slice order, then the point's gradients.  They
+// come first so that their loads overlap the column sums of the other blocks,
+// which sum one block of env-map-gradient columns each.
 template <int V, bool DIFF, bool SPEC>
-__global__ __launch_bounds__(256) void reduce_partials_kernel(Problem pr, Tiling tl,
-                                                              int64_t nrows, int ncol, int R,
-                                                              int ncomb) {
-  if ((int)blockIdx.y >= ncomb) {
-    reduce_rows_item(pr.part, pr.part2, nrows, ncol, R, blockIdx.y - ncomb, blockIdx.x);
+__global__ __launch_bounds__(1024) void envgrad_reduce_kernel(Problem pr, Tiling tl, int ncomb) {
+  const int64_t nrows = (pr.P + 63) / 64;
+  if ((int)blockIdx.x >= ncomb) {
+    reduce_cols_item(pr.part, pr.gE, nrows, pr.H, pr.W, (int)blockIdx.x - ncomb);
     return;
   }
-  if (blockIdx.x != 0) return;
-  const int64_t idx = (int64_t)blockIdx.y * 256 + threadIdx.x;
-  const int64_t ntail = nrows - tl.split0;  // >= 0 (checked at launch)
-  if (idx >= ntail * 64) return;
-  const int64_t k = idx >> 6;
-  const int lane = (int)(idx & 63);
+  if (threadIdx.x >= 64) return;
+  const int64_t k = blockIdx.x;  // tail tile k: tile split0 + k (< nrows, checked at launch)
+  const int lane = (int)threadIdx.x;
   const int64_t q = (tl.split0 + k) * 64 + lane;
   if (q >= pr.P) return;
   const int64_t p = pr.order != nullptr ? (int64_t)pr.order[q] : q;
@@ -949,39 +941,19 @@ __global__ __launch_bounds__(256) void reduce_partials_kernel(Problem pr, Tiling
   write_backward_outputs<V, DIFF, SPEC>(pr, st, p, gs, acc);
 }
 
-__global__ __launch_bounds__(256) void finish_envgrad_kernel(const double* __restrict__ part2,
-                                                             float* __restrict__ gE, int ngroups,
-                                                             int H, int W) {
-  finish_item(part2, gE, ngroups, H, W, blockIdx.x);
-}
-
-__global__ __launch_bounds__(256) void batch_reduce_partials_kernel(
+// The same column sums for every set of a batch, persistent blocks over the
+// (set, column block) items.
+__global__ __launch_bounds__(1024) void batch_envgrad_reduce_kernel(
     const BatchHeader* __restrict__ hdr, const BatchSet* __restrict__ sets, int32_t sig) {
   if (!hdr->has_backward || hdr->sig_bwd != sig) return;
-  const int64_t total = hdr->total_r1;
-  const int n = hdr->n_sets;
-  for (int64_t item = blockIdx.x; item < total; item += gridDim.x) {
-    const int s = find_set(n, item, [&](int i) { return sets[i].r1_item0; });
-    const int64_t local = item - sets[s].r1_item0;
-    const int cb = (int)(local % sets[s].colblk);
-    const int64_t g = local / sets[s].colblk;
-    const Problem& pr = sets[s].pr;
-    reduce_rows_item(pr.part, pr.part2, (pr.P + 63) / 64, 3 * pr.T, sets[s].tl_bwd.reduce_rows, g,
-                     cb);
-  }
-}
-
-__global__ __launch_bounds__(256) void batch_finish_envgrad_kernel(
-    const BatchHeader* __restrict__ hdr, const BatchSet* __restrict__ sets, int32_t sig) {
-  if (!hdr->has_backward || hdr->sig_bwd != sig) return;
-  const int64_t total = hdr->total_r2;
+  const int64_t total = hdr->total_red;
   const int n = hdr->n_sets;
   for (int64_t item = blockIdx.x; item < total; item += gridDim.x) {
     const int s = __builtin_amdgcn_readfirstlane(
-        find_set(n, item, [&](int i) { return sets[i].r2_item0; }));
-    const int cb = (int)(item - sets[s].r2_item0);
+        find_set(n, item, [&](int i) { return sets[i].red_item0; }));
+    const int cb = (int)(item - sets[s].red_item0);
     const Problem& pr = sets[s].pr;
-    finish_item(pr.part2, pr.gE, sets[s].ngroups, pr.H, pr.W, cb);
+    reduce_cols_item(pr.part, pr.gE, (pr.P + 63) / 64, pr.H, pr.W, cb);
   }
 }
 
@@ -1003,9 +975,8 @@ hipError_t launch_fwd(const Problem& pr, const Tiling& tl, hipStream_t stream) {
                      tl.lds_fwd, stream, pr, tl);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess || tl.split0 == tiles) return e;
-  const unsigned ncomb = (unsigned)(((tiles - tl.split0) * 64 + 255) / 256);
-  hipLaunchKernelGGL((forward_combine_kernel<V, DIFF, SPEC>), dim3(ncomb), dim3(256), 0, stream,
-                     pr, tl);
+  hipLaunchKernelGGL((forward_combine_kernel<V, DIFF, SPEC>), dim3((unsigned)(tiles - tl.split0)),
+                     dim3(64), 0, stream, pr, tl);
   return hipGetLastError();
 }
 
@@ -1075,14 +1046,13 @@ hipError_t launch_backward(int variant, bool diff, bool spec, const Problem& pr,
 namespace {
 
 template <int V, bool DIFF, bool SPEC>
-hipError_t launch_stage1(const Problem& pr, const Tiling& tl, hipStream_t stream) {
+hipError_t launch_reduce(const Problem& pr, const Tiling& tl, hipStream_t stream) {
   const int ncol = 3 * pr.T;
   const int64_t nrows = (pr.P + 63) / 64;
-  const int ngroups = (int)((nrows + tl.reduce_rows - 1) / tl.reduce_rows);
-  const int ncomb = (int)(((nrows - tl.split0) * 64 + 255) / 256);
-  const unsigned cb = (unsigned)((ncol + 255) / 256);
-  hipLaunchKernelGGL((reduce_partials_kernel<V, DIFF, SPEC>), dim3(cb, ngroups + ncomb), dim3(256),
-                     0, stream, pr, tl, nrows, ncol, tl.reduce_rows, ncomb);
+  const int ncomb = (int)(nrows - tl.split0);
+  const unsigned cb = (unsigned)((ncol + kReduceCols - 1) / kReduceCols);
+  hipLaunchKernelGGL((envgrad_reduce_kernel<V, DIFF, SPEC>), dim3(cb + ncomb),
+                     dim3(64 * kReduceWaves), 0, stream, pr, tl, ncomb);
   return hipGetLastError();
 }
 
@@ -1090,37 +1060,26 @@ hipError_t launch_stage1(const Problem& pr, const Tiling& tl, hipStream_t stream
 
 hipError_t launch_envgrad_reduce(int variant, bool diff, bool spec, const Problem& pr,
                                  const Tiling& tl, hipStream_t stream) {
-  const int ncol = 3 * pr.T;
   const int64_t nrows = (pr.P + 63) / 64;
-  const int ngroups = (int)((nrows + tl.reduce_rows - 1) / tl.reduce_rows);
   if (tl.split0 > nrows || tl.slices < 1) return hipErrorInvalidValue;
-  hipError_t e;
   switch (variant) {
-    case kDiffuse: e = launch_stage1<kDiffuse, true, false>(pr, tl, stream); break;
+    case kDiffuse: return launch_reduce<kDiffuse, true, false>(pr, tl, stream);
     case kSpecular:
-      if (diff && spec) e = launch_stage1<kSpecular, true, true>(pr, tl, stream);
-      else if (diff) e = launch_stage1<kSpecular, true, false>(pr, tl, stream);
-      else if (spec) e = launch_stage1<kSpecular, false, true>(pr, tl, stream);
-      else e = launch_stage1<kSpecular, false, false>(pr, tl, stream);
-      break;
+      if (diff && spec) return launch_reduce<kSpecular, true, true>(pr, tl, stream);
+      if (diff) return launch_reduce<kSpecular, true, false>(pr, tl, stream);
+      if (spec) return launch_reduce<kSpecular, false, true>(pr, tl, stream);
+      return launch_reduce<kSpecular, false, false>(pr, tl, stream);
     case kClamp:
-      if (diff && spec) e = launch_stage1<kClamp, true, true>(pr, tl, stream);
-      else if (diff) e = launch_stage1<kClamp, true, false>(pr, tl, stream);
-      else if (spec) e = launch_stage1<kClamp, false, true>(pr, tl, stream);
-      else e = launch_stage1<kClamp, false, false>(pr, tl, stream);
-      break;
+      if (diff && spec) return launch_reduce<kClamp, true, true>(pr, tl, stream);
+      if (diff) return launch_reduce<kClamp, true, false>(pr, tl, stream);
+      if (spec) return launch_reduce<kClamp, false, true>(pr, tl, stream);
+      return launch_reduce<kClamp, false, false>(pr, tl, stream);
     case kV2:
-      e = spec ? launch_stage1<kV2, true, true>(pr, tl, stream) : launch_stage1<kV2, true, false>(pr, tl, stream);
-      break;
+      return spec ? launch_reduce<kV2, true, true>(pr, tl, stream) : launch_reduce<kV2, true, false>(pr, tl, stream);
     case kVis:
-      e = spec ? launch_stage1<kVis, true, true>(pr, tl, stream) : launch_stage1<kVis, true, false>(pr, tl, stream);
-      break;
+      return spec ? launch_reduce<kVis, true, true>(pr, tl, stream) : launch_reduce<kVis, true, false>(pr, tl, stream);
     default: return hipErrorInvalidValue;
   }
-  if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(finish_envgrad_kernel, dim3((unsigned)((ncol + 15) / 16)), dim3(256), 0,
-                     stream, pr.part2, pr.gE, ngroups, pr.H, pr.W);
-  return hipGetLastError();
 }
 
 hipError_t launch_fill_zero(float* x, int64_t n, hipStream_t stream) {
@@ -1192,12 +1151,8 @@ hipError_t launch_batch_backward(int variant, bool diff, bool spec, const BatchL
   if (e != hipSuccess) return e;
   e = dispatch_batch<false>(variant, diff, spec, bl, stream);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(batch_reduce_partials_kernel, dim3(bl.reduce_grid), dim3(256), 0, stream,
-                     bl.hdr, bl.sets, bl.sig);
-  e = hipGetLastError();
-  if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(batch_finish_envgrad_kernel, dim3(bl.reduce_grid), dim3(256), 0, stream,
-                     bl.hdr, bl.sets, bl.sig);
+  hipLaunchKernelGGL(batch_envgrad_reduce_kernel, dim3(bl.reduce_grid), dim3(64 * kReduceWaves), 0,
+                     stream, bl.hdr, bl.sets, bl.sig);
   return hipGetLastError();
 }
 

@@ -62,7 +62,6 @@ class Request:
         return None if self.done_s is None else self.done_s - self.arrival_s
 
 
-REDUCE_ROWS = 16  # partial rows per stage-1 group of the env-gradient reduce
 ORDER_MIN_PAIRS = 1 << 24  # sets of this many point-texel pairs are sorted (both directions)
 ORDER_BLOCK = 1024  # points per block of the counting sort
 ORDER_KEYS = 1024  # keys (octahedral cells) of the counting sort
@@ -87,10 +86,9 @@ def request_parts(points: int, texels: int, backward: bool, variant: str = "vis"
     set large enough to be sorted), as the library lays them out."""
     out = points * 3 * 4 * (2 if variant in ("v2", "vis") else 1)
     tiles = (points + 63) // 64
-    # per-tile fp32 partials, then fp64 sums of REDUCE_ROWS tiles each, both
-    # 256-byte aligned (the library's batch workspace holds them for every set)
-    ws = _align256(tiles * 3 * texels * 4) + \
-        _align256(((tiles + REDUCE_ROWS - 1) // REDUCE_ROWS) * 3 * texels * 8)
+    # per-tile fp32 partials, 256-byte aligned (the library's batch workspace
+    # holds them for every set)
+    ws = _align256(tiles * 3 * texels * 4)
     if points * texels >= ORDER_MIN_PAIRS:
         ws += _order_bytes(points)
     grads = points * (3 + 3 + 3 + 3 + 1) * 4 + 3 * texels * 4 if backward else 0

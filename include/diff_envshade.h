@@ -225,9 +225,9 @@ DES_API int des_set_tiling_override(int32_t ntc_forward, int32_t ntc_backward);
  * per-device scratch array of des_static_scratch_bytes() bytes (32 MiB) that
  * the library holds from its load to the end of the process.  Problems whose
  * des_backward_workspace_size exceeds it are rejected with
- * DES_ERR_INVALID_ARGUMENT: every problem of up to 223 040 points with a
- * 16 x 32 map, or up to 71 808 points with a 32 x 64 map, fits (70 000 points
- * at 16 x 32 use 9.6 MB).
+ * DES_ERR_INVALID_ARGUMENT: every problem of up to 276 480 points with a
+ * 16 x 32 map, or up to 77 184 points with a 32 x 64 map, fits (70 000 points
+ * at 16 x 32 use 8.7 MB).
  * Nothing is allocated: the
  * first call on a device only looks up that device's scratch address.  All
  * calls on one GPU share its scratch, so calls that may overlap on one GPU

@@ -72,7 +72,10 @@ def test_noalloc_backward_rejects_problems_beyond_the_scratch():
     L = _lib.load()
     limit = L.des_static_scratch_bytes()
     assert limit == 32 << 20
-    P, H, W = 300000, 16, 32
+    # the documented limits: 276 480 points at 16 x 32, 77 184 at 32 x 64
+    for n, h, w in ((276480, 16, 32), (77184, 32, 64)):
+        assert _lib.workspace_size(n, h, w) <= limit < _lib.workspace_size(n + 1, h, w)
+    P, H, W = 400000, 16, 32
     assert _lib.workspace_size(P, H, W) > limit
     args = [P, 3, H, W, 1, 1, 16] + [4096] * 13 + [1] + [4096] * 6 + [None]
     rc = L.des_bruteforce_specularvisibility_shader_backward2_noalloc(*args)
