@@ -20,12 +20,14 @@
 //   1. key of each point and its rank among the equal-key points of its block
 //      that precede it (within a wave: a match mask built from one ballot per
 //      key bit; across waves and rounds: per-key counts in LDS), plus the
-//      block's per-key counts, stored key-major: count[key][block];
-//   2. (more than 8 blocks only) per key, an exclusive scan of count[key][.]
-//      over blocks in place, and the key totals;
+//      block's per-key counts: up to kFusedBlocks blocks block-major,
+//      count[block][key], beyond that key-major, count[key][block];
+//   2. (more than kFusedBlocks blocks only) per key, an exclusive scan of
+//      count[key][.] over blocks in place, and the key totals;
 //   3. key bases (an exclusive scan over keys of the totals, in LDS) and the
 //      scatter: each point goes to base[key] + prefix[key][block] + rank.  Up
-//      to 8 blocks, each block sums the count rows itself (step 2 fused).
+//      to kFusedBlocks blocks, each block reads every block's count row
+//      itself (coalesced 4 KB rows) and step 2 is not launched.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -41,9 +43,9 @@ constexpr int kWaves = kThreads / 64;             // 16
 constexpr int kKeys = kOrderKeys;                 // 1024
 constexpr int kKeyBits = 10;
 // Up to this many blocks each scatter block sums the count rows itself (that
-// work grows with the square of the block count); beyond, a separate per-key
-// scan runs first.
-constexpr int kFusedBlocks = 8;
+// work grows with the square of the block count; 128 blocks are 512 KB read
+// per block); beyond, a separate per-key scan runs first.
+constexpr int kFusedBlocks = 128;
 
 // Hilbert-curve index of cell (x, y) of a 32 x 32 grid.  Consecutive indices
 // are always neighbouring cells, so a run of 64 sorted points stays compact
@@ -182,10 +184,12 @@ __device__ __forceinline__ void rank_block(const SortJob& job, int blk) {
     const int64_t p = (int64_t)blk * kOrderBlock + r * kThreads + tid;
     if (p < P) keyrank[p] = kr[r];
   }
-  uint32_t* row = count + (int64_t)tid * nblk_pad;
-  row[blk] = run[tid];
-  if (blk == nblk - 1)
-    for (int j = nblk; j < nblk_pad; ++j) row[j] = 0u;  // row padding read by the fused scan
+  if (job.fused) {
+    count[(int64_t)blk * kKeys + tid] = run[tid];  // block-major: the scatter reads whole rows
+  } else {
+    count[(int64_t)tid * nblk_pad + blk] = run[tid];
+  }
+  (void)nblk;
 }
 
 // inclusive scan over the 64 lanes of a wave
@@ -219,9 +223,9 @@ __device__ __forceinline__ void scan_group(const SortJob& job, int group) {
   if (lane == 0) total[key] = carry;
 }
 
-// 3. key bases, then scatter.  fused: count holds the raw per-block counts
-// (rows padded with zeros to nblk_pad, a multiple of 4); otherwise the
-// exclusive prefixes of step 2 and total[] the key totals.
+// 3. key bases, then scatter.  fused: count holds the raw per-block counts,
+// block-major; otherwise the key-major exclusive prefixes of step 2 and
+// total[] the key totals.
 __device__ __forceinline__ void scatter_block(const SortJob& job, int blk) {
   const uint32_t* __restrict__ keyrank = job.keyrank;
   const uint32_t* __restrict__ count = job.count;
@@ -243,13 +247,25 @@ __device__ __forceinline__ void scatter_block(const SortJob& job, int blk) {
   }
   uint32_t tot = 0u, pre = 0u;
   if (fused) {
-    const uint4* row = reinterpret_cast<const uint4*>(count + (int64_t)tid * nblk_pad);
-    for (int j4 = 0; j4 < (nblk_pad >> 2); ++j4) {
-      const uint4 c = row[j4];
-      const int j = 4 * j4;
-      tot += (c.x + c.y) + (c.z + c.w);
-      pre += (j < blk ? c.x : 0u) + (j + 1 < blk ? c.y : 0u) + (j + 2 < blk ? c.z : 0u) +
-             (j + 3 < blk ? c.w : 0u);
+    // key tid's count in every block: one 4 KB row per block, read whole by
+    // the workgroup (eight rows in flight)
+    const uint32_t* col = count + tid;
+    const int nblk = job.nblk;
+    int j = 0;
+    for (; j + 8 <= nblk; j += 8) {
+      uint32_t c[8];
+#pragma unroll
+      for (int i = 0; i < 8; ++i) c[i] = col[(int64_t)(j + i) * kKeys];
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        tot += c[i];
+        pre += (j + i < blk) ? c[i] : 0u;
+      }
+    }
+    for (; j < nblk; ++j) {
+      const uint32_t c = col[(int64_t)j * kKeys];
+      tot += c;
+      pre += (j < blk) ? c : 0u;
     }
   } else {
     tot = total[tid];
