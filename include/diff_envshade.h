@@ -83,7 +83,11 @@ DES_API size_t des_forward_workspace_size(int64_t P, int32_t H, int32_t W);
 /* Point-order policy (no counterpart in the reference), process-wide:
  * 0 = automatic (both directions reorder from 2^24 point-texel pairs
  * P x H x W), 1 = always, -1 = never.
- * The workspace sizes above follow the policy in force when they are asked. */
+ * The workspace sizes above follow the policy in force when they are asked.
+ * Process-wide on purpose: PyTorch's autograd runs a graph's backward on its
+ * own thread, so a per-thread policy would shade the forward and the backward
+ * of one graph under different settings.  Results move with it at fp32
+ * rounding level only; set it before any thread shades, not while they do. */
 DES_API int des_set_point_order(int32_t mode);
 
 /* Point order shared between calls (no counterpart in the reference).  The
@@ -214,7 +218,8 @@ DES_API int des_bruteforce_specularvisibility_shader_backward2(
 /* Launch-shape override for tuning (no counterpart in the reference): number
  * of waves splitting the texel range per 64-point tile, forward and backward
  * (each a power of two, forward <= 16, backward <= 8; 0 restores the built-in
- * choice).  Process-wide; results do not depend on it beyond fp32 rounding. */
+ * choice).  Process-wide, like des_set_point_order and for the same reason;
+ * results do not depend on it beyond fp32 rounding. */
 DES_API int des_set_tiling_override(int32_t ntc_forward, int32_t ntc_backward);
 
 /* ---- backward without allocation (no counterpart in the reference) ---------
