@@ -904,7 +904,7 @@ DES_API int des_batch_prepare(int32_t variant, int enable_diffuse, int enable_sp
     b.tile0 = tile0;
     b.red_item0 = red;
     tile0 += tiles;
-    red += (3 * pr.T + 15) / 16;
+    red += (3 * pr.T + 63) / 64;  // batched reduce: 64-column items
   }
   hdr->n_sets = n_sets;
   hdr->variant = variant;

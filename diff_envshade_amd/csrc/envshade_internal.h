@@ -73,7 +73,7 @@ struct BatchSet {
   Tiling tl_fwd;      // ntc equals the forward launch's ntc; cw/st follow this set's T
   Tiling tl_bwd;      // same for the backward launch
   int64_t tile0;      // first global tile of this set
-  int64_t red_item0;  // first env-gradient reduce item (one per 16 columns of 3 T)
+  int64_t red_item0;  // first env-gradient reduce item (one per 64 columns of 3 T)
 };
 
 // Launch signature of a prepared batch: variant, branch flags and the waves per

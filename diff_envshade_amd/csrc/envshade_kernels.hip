@@ -854,46 +854,75 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(3))) void b
 // ---- env-map gradient: fixed-order FP64 reduction of the per-tile partials -----
 
 // gE[c][t] = dOmega_t * sum over the tile rows of part[row][c*T + t], in FP64,
-// rounded once to fp32.  One 1024-thread block owns 16 columns: lane l of
-// wave w reads column (l & 15) of the rows r0_w + (l >> 4) + 4 i, where wave w
-// covers rows [w*nrows/16, (w+1)*nrows/16) -- four 64-byte row segments per
-// load, every load of a lane issued before the first add.  The four partial
-// sums of a column in a wave are added in sub-row order, then wave 0 adds the
-// 16 wave sums in order, so the FP64 sum order depends only on nrows.
+// rounded once to fp32, in one fixed order: wave w of a 1024-thread block
+// covers rows [w*nrows/16, (w+1)*nrows/16) of a block of columns; within it
+// the rows r with (r - r0) mod 4 = k are summed in order into sum k, the four
+// sums are added as ((s0 + s1) + s2) + s3, and wave 0 adds the 16 wave sums in
+// order.  The order depends only on nrows, so grad_envmap is bitwise
+// reproducible, and the two thread mappings below give the same bits:
+//   COLS = 16: lane l reads column l & 15 of the rows r0 + (l >> 4) + 4 i --
+//     four 64-byte row segments per load, 96 blocks at the 16x32 headline
+//     (latency-bound: many blocks, eight loads in flight per lane);
+//   COLS = 64: lane l reads column l of every row, keeping the four sums --
+//     one 256-byte row segment per load, for the batched launches whose
+//     partials run to hundreds of MB (bandwidth-bound).
 constexpr int kReduceWaves = 16;
-constexpr int kReduceCols = 16;
+constexpr int kReduceCols = 16;       // single-problem launches
+constexpr int kReduceColsBatch = 64;  // batched launches
 
+template <int COLS>
 __device__ __forceinline__ void reduce_cols_item(const float* __restrict__ part,
                                                  float* __restrict__ gE, int64_t nrows, int H,
                                                  int W, int cb) {
-  __shared__ double s_sum[kReduceWaves][kReduceCols];
+  static_assert(COLS == 16 || COLS == 64, "column block");
+  __shared__ double s_sum[kReduceWaves][64];
   const int T = H * W;
   const int ncol = 3 * T;
   const int lane = threadIdx.x & 63;
   const int w = threadIdx.x >> 6;
-  const int j = lane & 15;
-  const int rs = lane >> 4;
-  const int col = cb * kReduceCols + j;
+  const int j = (COLS == 16) ? (lane & 15) : lane;
+  const int col = cb * COLS + j;
   const int64_t r0 = nrows * w / kReduceWaves;
   const int64_t r1 = nrows * (w + 1) / kReduceWaves;
-  double s = 0.0;
-  if (col < ncol) {
-    const float* src = part + col;
-    int64_t r = r0 + rs;
-    for (; r + 28 < r1; r += 32) {
-      float v[8];
+  double tot = 0.0;
+  if (COLS == 16) {
+    const int rs = lane >> 4;
+    double s = 0.0;
+    if (col < ncol) {
+      const float* src = part + col;
+      int64_t r = r0 + rs;
+      for (; r + 28 < r1; r += 32) {
+        float v[8];
 #pragma unroll
-      for (int i = 0; i < 8; ++i) v[i] = src[(r + 4 * i) * ncol];
+        for (int i = 0; i < 8; ++i) v[i] = src[(r + 4 * i) * ncol];
 #pragma unroll
-      for (int i = 0; i < 8; ++i) s += (double)v[i];
+        for (int i = 0; i < 8; ++i) s += (double)v[i];
+      }
+      for (; r < r1; r += 4) s += (double)src[r * ncol];
     }
-    for (; r < r1; r += 4) s += (double)src[r * ncol];
+    // sub-rows 0..3 of each column, in order: lanes j, j+16, j+32, j+48
+    const double s1 = __shfl(s, j + 16, 64), s2 = __shfl(s, j + 32, 64);
+    const double s3 = __shfl(s, j + 48, 64);
+    tot = ((s + s1) + s2) + s3;
+  } else {
+    double s[4] = {0.0, 0.0, 0.0, 0.0};
+    if (col < ncol) {
+      const float* src = part + col;
+      int64_t r = r0;
+      for (; r + 8 <= r1; r += 8) {
+        float v[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) v[i] = src[(r + i) * ncol];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) s[i & 3] += (double)v[i];  // r - r0 = 8 k + i
+      }
+      for (; r < r1; ++r) s[(r - r0) & 3] += (double)src[r * ncol];
+    }
+    tot = ((s[0] + s[1]) + s[2]) + s[3];
   }
-  // sub-rows 0..3 of each column, in order: lanes j, j+16, j+32, j+48
-  const double s1 = __shfl(s, j + 16, 64), s2 = __shfl(s, j + 32, 64), s3 = __shfl(s, j + 48, 64);
-  if (rs == 0) s_sum[w][j] = ((s + s1) + s2) + s3;
+  if (COLS == 64 || lane < 16) s_sum[w][j] = tot;
   __syncthreads();
-  if (w == 0 && rs == 0 && col < ncol) {
+  if (w == 0 && (COLS == 64 || lane < 16) && col < ncol) {
     double t = 0.0;
 #pragma unroll
     for (int k = 0; k < kReduceWaves; ++k) t += s_sum[k][j];
@@ -915,7 +944,7 @@ template <int V, bool DIFF, bool SPEC>
 __global__ __launch_bounds__(1024) void envgrad_reduce_kernel(Problem pr, Tiling tl, int ncomb) {
   const int64_t nrows = (pr.P + 63) / 64;
   if ((int)blockIdx.x >= ncomb) {
-    reduce_cols_item(pr.part, pr.gE, nrows, pr.H, pr.W, (int)blockIdx.x - ncomb);
+    reduce_cols_item<kReduceCols>(pr.part, pr.gE, nrows, pr.H, pr.W, (int)blockIdx.x - ncomb);
     return;
   }
   if (threadIdx.x >= 64) return;
@@ -953,7 +982,7 @@ __global__ __launch_bounds__(1024) void batch_envgrad_reduce_kernel(
         find_set(n, item, [&](int i) { return sets[i].red_item0; }));
     const int cb = (int)(item - sets[s].red_item0);
     const Problem& pr = sets[s].pr;
-    reduce_cols_item(pr.part, pr.gE, (pr.P + 63) / 64, pr.H, pr.W, cb);
+    reduce_cols_item<kReduceColsBatch>(pr.part, pr.gE, (pr.P + 63) / 64, pr.H, pr.W, cb);
   }
 }
 
