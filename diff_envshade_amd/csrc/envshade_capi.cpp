@@ -96,8 +96,10 @@ des::Tiling tiling_for(int ntc, int T, bool backward) {
   const size_t table = size_t(des::kTexelBytes) * tl.st;
   const size_t staged = table + size_t(tl.st / 32) * 3 * 64 * sizeof(uint32_t);
   tl.vis_off = int(table);
-  tl.lds_fwd = std::max(staged, size_t(ntc) * des::kForwardAcc * 64 * sizeof(float));
-  tl.lds_bwd = std::max(staged, size_t(ntc) * des::kBackwardAcc * 64 * sizeof(float));
+  // (the tile's exchanged point fields also use the space from offset 0, first)
+  const size_t xch = size_t(des::kPointXchSlots) * 64 * sizeof(float);
+  tl.lds_fwd = std::max({staged, xch, size_t(ntc) * des::kForwardAcc * 64 * sizeof(float)});
+  tl.lds_bwd = std::max({staged, xch, size_t(ntc) * des::kBackwardAcc * 64 * sizeof(float)});
   tl.split0 = INT64_MAX;  // set per problem by choose_tiling (single-set launches only)
   tl.slices = 1;
   tl.slice_texels = T;

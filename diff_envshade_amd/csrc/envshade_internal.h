@@ -119,6 +119,7 @@ constexpr int kForwardAcc = 6;
 constexpr int kBackwardAcc = 16;
 constexpr int kMaxSuperChunk = 512;  // texels staged in LDS at once (48 B each, 24 KB)
 constexpr int kTexelBytes = 48;
+constexpr int kPointXchSlots = 25;  // per-point fields a tile exchanges through LDS
 
 hipError_t launch_forward(int variant, bool diff, bool spec, const Problem& pr, const Tiling& tl,
                           hipStream_t stream);

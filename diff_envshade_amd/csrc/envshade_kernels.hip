@@ -158,6 +158,96 @@ __device__ __forceinline__ void load_point(const Problem& pr, int64_t p, bool va
   }
 }
 
+// ---- the tile's point data, gathered once per tile ------------------------------
+//
+// Every wave of a tile shades the same 64 points, and in sorted order each
+// point's fields are scattered over the arrays: a load instruction touches 64
+// cache lines, and the tile's start is bound by how many of them the CU's
+// vector memory path can take.  So each field group is gathered by one wave
+// (group g by wave g mod ntc), written to LDS (slot-major, [slot][lane]), and
+// read by every wave after a barrier.  Slots:
+//   0-2 normal, 3-5 view_dir, 6 roughness, 7-9 the face's vertex ids (as int
+//   bits; 0 with zero weights where the face or a vertex id is out of range),
+//   10-12 barycentric weights; backward: 13-15 grad_shading, 16-18 albedo,
+//   19-21 grad_diffuse_shading, 22-24 specular albedo.
+// kPointXchSlots x 64 floats at LDS offset 0, before the texel table is built
+// there.
+template <int V, bool DIFF, bool SPEC, bool BWD>
+__device__ __forceinline__ void gather_point(const Problem& pr, int64_t p, bool valid, int wv,
+                                             int ntc, int lane, float* xch) {
+  constexpr bool DOUT = (V == kV2 || V == kVis);
+  auto put = [&](int slot, float x) { xch[slot * 64 + lane] = x; };
+  auto put3 = [&](int slot, const float* a) {
+    put(slot, valid ? a[3 * p] : 0.f);
+    put(slot + 1, valid ? a[3 * p + 1] : 0.f);
+    put(slot + 2, valid ? a[3 * p + 2] : 0.f);
+  };
+  constexpr int NG = BWD ? 5 : 3;
+  for (int g = wv; g < NG; g += ntc) {
+    if (g == 0) {
+      put3(0, pr.normal);
+      put(6, (SPEC && valid) ? pr.rough[p] : 0.f);
+    } else if (g == 1) {
+      if (SPEC || V == kClamp) put3(3, pr.view_dir);
+    } else if (g == 2) {
+      if (V == kVis) {
+        // bs:2145-2151: nearest triangle -> 3 vertices, barycentric weights (3 x P)
+        int a = 0, b = 0, c = 0;
+        float c0 = 0.f, c1 = 0.f, c2 = 0.f;
+        if (valid) {
+          const float b0 = pr.bary[p], b1 = pr.bary[pr.P + p], b2 = pr.bary[2 * pr.P + p];
+          const int64_t f = pr.tri[p];
+          if (f >= 0 && f < (int64_t)pr.N_face) {
+            const int fa = pr.fvl[f], fb = pr.fvl[(int64_t)pr.N_face + f];
+            const int fc = pr.fvl[2 * (int64_t)pr.N_face + f];
+            // out-of-range ids read as occluded instead of faulting
+            if (fa >= 0 && fa < pr.N_vertex && fb >= 0 && fb < pr.N_vertex && fc >= 0 &&
+                fc < pr.N_vertex) {
+              a = fa; b = fb; c = fc;
+              c0 = b0; c1 = b1; c2 = b2;
+            }
+          }
+        }
+        put(7, __int_as_float(a));
+        put(8, __int_as_float(b));
+        put(9, __int_as_float(c));
+        put(10, c0);
+        put(11, c1);
+        put(12, c2);
+      }
+    } else if (g == 3) {
+      put3(13, pr.g_sh);
+      if (DIFF) put3(16, pr.albedo);
+    } else {
+      if (DIFF && DOUT) put3(19, pr.g_ds);
+      if (SPEC) put3(22, pr.spec);
+    }
+  }
+}
+
+// The point state from the exchanged fields (as load_point builds it).
+template <int V, bool SPEC>
+__device__ __forceinline__ void read_point(const float* xch, bool valid, int lane,
+                                           PointState<V, SPEC>& s) {
+  auto get = [&](int slot) { return xch[slot * 64 + lane]; };
+  const bool hv = SPEC || V == kClamp;
+  static_cast<PointGeom&>(s) = point_geom(get(0), get(1), get(2), hv ? get(3) : 0.f,
+                                          hv ? get(4) : 0.f, hv ? get(5) : 0.f,
+                                          SPEC ? get(6) : 0.f);
+  s.live = valid ? 1.f : 0.f;
+  if (V == kClamp && !(s.nv > 0.f)) s.live = 0.f;  // per-point back-face cull (bs:1111)
+  s.vert0 = s.vert1 = s.vert2 = 0;
+  s.c0 = s.c1 = s.c2 = 0.f;
+  if (V == kVis) {
+    s.vert0 = __float_as_int(get(7));
+    s.vert1 = __float_as_int(get(8));
+    s.vert2 = __float_as_int(get(9));
+    s.c0 = get(10);
+    s.c1 = get(11);
+    s.c2 = get(12);
+  }
+}
+
 // Visibility words of one 32-texel slot (bs:2180-2185): word for vertex v of
 // slot s at visibility[s * N_vertex + v].
 struct VisWords {
@@ -233,16 +323,25 @@ __device__ __forceinline__ float uniform(float x) {
   return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x)));
 }
 
+// Sum / minimum over the wave, the same in every lane: the 16 lanes of each
+// row combined with row-rotate DPP, then the four rows' values (lanes 0, 16,
+// 32, 48) in order -- no LDS round trips.
+__device__ __forceinline__ float lane_value(float x, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), l));
+}
+
 __device__ __forceinline__ float wave_sum(float x) {
-#pragma unroll
-  for (int d = 32; d >= 1; d >>= 1) x += __shfl_xor(x, d, 64);
-  return x;  // the same in every lane: each step adds the same two values
+  x = row_allreduce(x);
+  return ((lane_value(x, 0) + lane_value(x, 16)) + lane_value(x, 32)) + lane_value(x, 48);
 }
 
 __device__ __forceinline__ float wave_min(float x) {
-#pragma unroll
-  for (int d = 32; d >= 1; d >>= 1) x = fminf(x, __shfl_xor(x, d, 64));
-  return x;
+  x = fminf(x, dpp<0x128>(x));  // row_ror:8
+  x = fminf(x, dpp<0x124>(x));  // row_ror:4
+  x = fminf(x, dpp<0x122>(x));  // row_ror:2
+  x = fminf(x, dpp<0x121>(x));  // row_ror:1
+  return fminf(fminf(lane_value(x, 0), lane_value(x, 16)),
+               fminf(lane_value(x, 32), lane_value(x, 48)));
 }
 
 // Cone of the normals of the lanes that can contribute (live, finite, non-zero).
@@ -423,7 +522,12 @@ __device__ __forceinline__ void forward_tile(const Problem& pr, const Tiling& tl
   const int64_t p = (order != nullptr && valid) ? (int64_t)order[q] : q;
 
   PointState<V, SPEC> st;
-  load_point<V, SPEC>(pr, p, valid, st);
+  {
+    float* xch = reinterpret_cast<float*>(smem);
+    gather_point<V, DIFF, SPEC, false>(pr, p, valid, wv, tl.ntc, lane, xch);
+    __syncthreads();
+    read_point<V, SPEC>(xch, valid, lane, st);
+  }
   bool single = true;
   float csum = 0.f;
   if (V == kVis) {
@@ -664,25 +768,28 @@ __device__ __forceinline__ void backward_tile(const Problem& pr, const Tiling& t
   const int64_t p = (order != nullptr && valid) ? (int64_t)order[q0] : q0;
 
   PointState<V, SPEC> st;
-  load_point<V, SPEC>(pr, p, valid, st);
+  float gs[3] = {0.f, 0.f, 0.f}, A[3] = {0.f, 0.f, 0.f}, B[3] = {0.f, 0.f, 0.f};
+  {
+    float* xch = reinterpret_cast<float*>(smem);
+    gather_point<V, DIFF, SPEC, true>(pr, p, valid, wv, tl.ntc, lane, xch);
+    __syncthreads();
+    read_point<V, SPEC>(xch, valid, lane, st);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      gs[c] = xch[(13 + c) * 64 + lane];
+      if (DIFF) {
+        float up = gs[c] * xch[(16 + c) * 64 + lane];
+        if (DOUT) up += xch[(19 + c) * 64 + lane];
+        A[c] = up * kInvPi;
+      }
+      if (SPEC) B[c] = gs[c] * xch[(22 + c) * 64 + lane];
+    }
+  }
   bool single = true;
   float csum = 0.f;
   if (V == kVis) {
     single = __all(st.vert0 == st.vert1 && st.vert1 == st.vert2);
     csum = st.c0 + st.c1 + st.c2;
-  }
-  float gs[3] = {0.f, 0.f, 0.f}, A[3] = {0.f, 0.f, 0.f}, B[3] = {0.f, 0.f, 0.f};
-  if (valid) {
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      gs[c] = pr.g_sh[3 * p + c];
-      if (DIFF) {
-        float up = gs[c] * pr.albedo[3 * p + c];
-        if (DOUT) up += pr.g_ds[3 * p + c];
-        A[c] = up * kInvPi;
-      }
-      if (SPEC) B[c] = gs[c] * pr.spec[3 * p + c];
-    }
   }
 
   const Cone cone = tile_cone(st);
