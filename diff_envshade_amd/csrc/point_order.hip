@@ -20,14 +20,14 @@
 //   1. key of each point and its rank among the equal-key points of its block
 //      that precede it (within a wave: a match mask built from one ballot per
 //      key bit; across waves and rounds: per-key counts in LDS), plus the
-//      block's per-key counts: up to kFusedBlocks blocks block-major,
-//      count[block][key], beyond that key-major, count[key][block];
+//      block's per-key counts: up to kFusedBlocks blocks block-major as 16-bit
+//      words, count[block][key], beyond that key-major, count[key][block];
 //   2. (more than kFusedBlocks blocks only) per key, an exclusive scan of
 //      count[key][.] over blocks in place, and the key totals;
 //   3. key bases (an exclusive scan over keys of the totals, in LDS) and the
 //      scatter: each point goes to base[key] + prefix[key][block] + rank.  Up
 //      to kFusedBlocks blocks, each block reads every block's count row
-//      itself (coalesced 4 KB rows) and step 2 is not launched.
+//      itself (coalesced 2 KB rows) and step 2 is not launched.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -46,6 +46,7 @@ constexpr int kKeyBits = 10;
 // work grows with the square of the block count; 128 blocks are 512 KB read
 // per block); beyond, a separate per-key scan runs first.
 constexpr int kFusedBlocks = 128;
+static_assert(kOrderBlock <= 0xffff, "a block's per-key count fits 16 bits");
 
 // Hilbert-curve index of cell (x, y) of a 32 x 32 grid.  Consecutive indices
 // are always neighbouring cells, so a run of 64 sorted points stays compact
@@ -185,7 +186,8 @@ __device__ __forceinline__ void rank_block(const SortJob& job, int blk) {
     if (p < P) keyrank[p] = kr[r];
   }
   if (job.fused) {
-    count[(int64_t)blk * kKeys + tid] = run[tid];  // block-major: the scatter reads whole rows
+    // block-major 16-bit counts (at most kOrderBlock per key): the scatter reads whole rows
+    reinterpret_cast<uint16_t*>(count)[(int64_t)blk * kKeys + tid] = run[tid];
   } else {
     count[(int64_t)tid * nblk_pad + blk] = run[tid];
   }
@@ -249,13 +251,13 @@ __device__ __forceinline__ void scatter_block(const SortJob& job, int blk) {
   if (fused) {
     // key tid's count in every block: one 4 KB row per block, read whole by
     // the workgroup (eight rows in flight)
-    const uint32_t* col = count + tid;
+    const uint16_t* col = reinterpret_cast<const uint16_t*>(count) + tid;
     const int nblk = job.nblk;
     int j = 0;
     for (; j + 8 <= nblk; j += 8) {
       uint32_t c[8];
 #pragma unroll
-      for (int i = 0; i < 8; ++i) c[i] = col[(int64_t)(j + i) * kKeys];
+      for (int i = 0; i < 8; ++i) c[i] = col[(int64_t)(j + i) * kKeys];  // widened
 #pragma unroll
       for (int i = 0; i < 8; ++i) {
         tot += c[i];
@@ -263,7 +265,7 @@ __device__ __forceinline__ void scatter_block(const SortJob& job, int blk) {
       }
     }
     for (; j < nblk; ++j) {
-      const uint32_t c = col[(int64_t)j * kKeys];
+      const uint32_t c = col[(int64_t)j * kKeys];  // widened
       tot += c;
       pre += (j < blk) ? c : 0u;
     }
