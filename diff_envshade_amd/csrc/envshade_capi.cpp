@@ -111,24 +111,30 @@ des::Tiling tiling_for(int ntc, int T, bool backward) {
 // MI355X (256 CUs x four 4-wave tiles); the tiles beyond a multiple of that
 // would otherwise run as a fifth tile on some CUs while the others idle
 // (measured at 70 000 points: 70 CUs did 5 tiles, the rest 4).  When that
-// remainder is at most half the resident count, its tiles are shaded as four
-// texel slices each, after the whole tiles, so every CU gets about the same
-// work.  The rule depends on (P, T) only, so results do not depend on the
-// device.
+// remainder is at most half the resident count, its tiles are shaded as texel
+// slices -- eight each in the backward when there are at most 128 of them,
+// else four, and four in the forward (measured best at 70 000 points: 8 in the
+// forward cost more set-up than it saved) -- after the
+// whole tiles, so every CU gets about the same work.  The rule depends on
+// (P, T) only, so results do not depend on the device.
 constexpr int64_t kResidentTiles = 1024;
-constexpr int kTailSlices = 4;
+constexpr int kTailSlicesForward = 4;
+constexpr int kTailSlicesBackward = 8;
 
 struct TailSplit {
   int64_t split0;
   int slices, slice_texels;
 };
 
-TailSplit tail_split(int64_t P, int T) {
+TailSplit tail_split(int64_t P, int T, bool backward) {
   const int64_t tiles = n_tiles(P);
   const int nslot = (T + 31) / 32;
   const int64_t rem = tiles % kResidentTiles;
-  if (tiles > kResidentTiles && rem > 0 && 2 * rem <= kResidentTiles && nslot >= kTailSlices)
-    return TailSplit{tiles - rem, kTailSlices, ((nslot + kTailSlices - 1) / kTailSlices) * 32};
+  // eight backward slices only for a short tail (a long one balances with four,
+  // and its per-point sums would grow the workspace eightfold)
+  const int k = (backward && rem <= kResidentTiles / 8) ? kTailSlicesBackward : kTailSlicesForward;
+  if (tiles > kResidentTiles && rem > 0 && 2 * rem <= kResidentTiles && nslot >= k)
+    return TailSplit{tiles - rem, k, ((nslot + k - 1) / k) * 32};
   return TailSplit{tiles, 1, T};
 }
 
@@ -155,7 +161,7 @@ des::Tiling choose_tiling(int64_t P, int T, bool backward) {
   des::Tiling tl = tiling_for(ntc, T, backward);
   tl.split0 = tiles;
   if (backward && ov <= 0 && ntc == 4) {
-    const TailSplit ts = tail_split(P, T);
+    const TailSplit ts = tail_split(P, T, true);
     tl.split0 = ts.split0;
     tl.slices = ts.slices;
     tl.slice_texels = ts.slice_texels;
@@ -188,7 +194,7 @@ WsLayout ws_layout(int64_t P, int T, bool allow_order) {
   w.part_off = 0;
   w.total = align256(size_t(tiles) * 3 * T * sizeof(float));
   if (allow_order) {
-    const TailSplit ts = tail_split(P, T);
+    const TailSplit ts = tail_split(P, T, true);
     w.spart_off = w.total;
     w.total += align256(size_t(tiles - ts.split0) * ts.slices * des::kBackwardAcc * 64 *
                         sizeof(float));
@@ -243,7 +249,7 @@ struct FwdLayout {
 FwdLayout fwd_layout(int64_t P, int T) {
   FwdLayout f;
   f.spart_off = align256(des::order_layout(P).total);
-  const TailSplit ts = tail_split(P, T);
+  const TailSplit ts = tail_split(P, T, false);
   f.total = f.spart_off +
             align256(size_t(n_tiles(P) - ts.split0) * ts.slices * des::kForwardAcc * 64 *
                      sizeof(float));
@@ -276,7 +282,7 @@ int run_forward(int variant, bool diff, bool spec, des::Problem& pr, void* ws, s
   des::Tiling tl = choose_tiling(pr.P, pr.T, false);
   // tail slices need the workspace for their per-point sums
   if (use_ws && tl.ntc == 4 && g_ntc_fwd_override.load() <= 0) {
-    const TailSplit ts = tail_split(pr.P, pr.T);
+    const TailSplit ts = tail_split(pr.P, pr.T, false);
     tl.split0 = ts.split0;
     tl.slices = ts.slices;
     tl.slice_texels = ts.slice_texels;

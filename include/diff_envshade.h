@@ -232,7 +232,7 @@ DES_API int des_set_tiling_override(int32_t ntc_forward, int32_t ntc_backward);
  * des_backward_workspace_size exceeds it are rejected with
  * DES_ERR_INVALID_ARGUMENT: every problem of up to 276 480 points with a
  * 16 x 32 map, or up to 77 184 points with a 32 x 64 map, fits (70 000 points
- * at 16 x 32 use 8.7 MB).
+ * at 16 x 32 use 9.9 MB).
  * Nothing is allocated: the
  * first call on a device only looks up that device's scratch address.  All
  * calls on one GPU share its scratch, so calls that may overlap on one GPU
