@@ -109,6 +109,8 @@ def load() -> ctypes.CDLL:
         L.des_set_tiling_override.restype = _int
         L.des_set_backward_timing_events.argtypes = [_vp, _vp]
         L.des_set_backward_timing_events.restype = _int
+        L.des_set_forward_timing_events.argtypes = [_vp, _vp]
+        L.des_set_forward_timing_events.restype = _int
         L.des_reuse_point_order_once.argtypes = [_vp, _i64]
         L.des_reuse_point_order_once.restype = _int
         L.des_static_scratch_bytes.argtypes = []
@@ -167,6 +169,14 @@ def set_backward_timing_events(start, stop) -> None:
     h0 = start.cuda_event if hasattr(start, "cuda_event") else start
     h1 = stop.cuda_event if hasattr(stop, "cuda_event") else stop
     check(load().des_set_backward_timing_events(h0, h1))
+
+
+def set_forward_timing_events(start, stop) -> None:
+    """Record torch.cuda.Event / hipEvent_t handles around the next forward's
+    main kernel (see des_set_forward_timing_events)."""
+    h0 = start.cuda_event if hasattr(start, "cuda_event") else start
+    h1 = stop.cuda_event if hasattr(stop, "cuda_event") else stop
+    check(load().des_set_forward_timing_events(h0, h1))
 
 
 def set_tiling_override(ntc_forward: int = 0, ntc_backward: int = 0) -> None:

@@ -63,6 +63,8 @@ using des_capi::ok;
 
 thread_local hipEvent_t g_bwd_ev_start = nullptr;
 thread_local hipEvent_t g_bwd_ev_stop = nullptr;
+thread_local hipEvent_t g_fwd_ev_start = nullptr;
+thread_local hipEvent_t g_fwd_ev_stop = nullptr;
 std::atomic<int> g_ntc_fwd_override{0};
 std::atomic<int> g_ntc_bwd_override{0};
 std::atomic<int> g_order_mode{0};  // 0 automatic, 1 always, -1 never
@@ -259,6 +261,8 @@ FwdLayout fwd_layout(int64_t P, int T) {
 int run_forward(int variant, bool diff, bool spec, des::Problem& pr, void* ws, size_t ws_bytes,
                 hipStream_t stream) {
   const void* reuse = des_capi::take_reuse_order(pr.P);
+  hipEvent_t ev0 = g_fwd_ev_start, ev1 = g_fwd_ev_stop;
+  g_fwd_ev_start = g_fwd_ev_stop = nullptr;
   if (pr.P == 0) return ok();
   const bool ordered = use_order(pr.P, pr.T, false);
   const bool use_ws = ws != nullptr && ordered;
@@ -288,7 +292,7 @@ int run_forward(int variant, bool diff, bool spec, des::Problem& pr, void* ws, s
     tl.slice_texels = ts.slice_texels;
     pr.spart = reinterpret_cast<float*>(static_cast<char*>(ws) + fl.spart_off);
   }
-  hipError_t e = des::launch_forward(variant, diff, spec, pr, tl, stream);
+  hipError_t e = des::launch_forward(variant, diff, spec, pr, tl, stream, ev0, ev1);
   if (e != hipSuccess) return launch_error(e, "forward launch failed");
   return ok();
 }
@@ -586,6 +590,14 @@ DES_API int des_set_backward_timing_events(void* start, void* stop) {
     return fail(DES_ERR_INVALID_ARGUMENT, "pass both timing events or neither");
   g_bwd_ev_start = static_cast<hipEvent_t>(start);
   g_bwd_ev_stop = static_cast<hipEvent_t>(stop);
+  return ok();
+}
+
+DES_API int des_set_forward_timing_events(void* start, void* stop) {
+  if ((start == nullptr) != (stop == nullptr))
+    return fail(DES_ERR_INVALID_ARGUMENT, "pass both timing events or neither");
+  g_fwd_ev_start = static_cast<hipEvent_t>(start);
+  g_fwd_ev_stop = static_cast<hipEvent_t>(stop);
   return ok();
 }
 

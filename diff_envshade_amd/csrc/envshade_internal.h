@@ -121,8 +121,11 @@ constexpr int kMaxSuperChunk = 512;  // texels staged in LDS at once (48 B each,
 constexpr int kTexelBytes = 48;
 constexpr int kPointXchSlots = 25;  // per-point fields a tile exchanges through LDS
 
+// Main shading kernel, then (tail slices) the combine; ev_start / ev_stop, when
+// given, are recorded right before and after the main kernel.
 hipError_t launch_forward(int variant, bool diff, bool spec, const Problem& pr, const Tiling& tl,
-                          hipStream_t stream);
+                          hipStream_t stream, hipEvent_t ev_start = nullptr,
+                          hipEvent_t ev_stop = nullptr);
 hipError_t launch_backward(int variant, bool diff, bool spec, const Problem& pr, const Tiling& tl,
                            hipStream_t stream);
 // The env-map-gradient reduction and the combine of the backward's tail

@@ -1103,14 +1103,19 @@ __global__ __launch_bounds__(256) void fill_zero_kernel(float* __restrict__ x, i
 }
 
 template <int V, bool DIFF, bool SPEC>
-hipError_t launch_fwd(const Problem& pr, const Tiling& tl, hipStream_t stream) {
+hipError_t launch_fwd(const Problem& pr, const Tiling& tl, hipStream_t stream, hipEvent_t ev0,
+                      hipEvent_t ev1) {
   const int64_t tiles = (pr.P + 63) / 64;
   if (tl.split0 > tiles || tl.slices < 1) return hipErrorInvalidValue;
   const unsigned nblk = (unsigned)(tl.split0 + (tiles - tl.split0) * tl.slices);
+  hipError_t e;
+  if (ev0 != nullptr && (e = hipEventRecord(ev0, stream)) != hipSuccess) return e;
   hipLaunchKernelGGL((shade_forward_kernel<V, DIFF, SPEC>), dim3(nblk), dim3(64 * tl.ntc),
                      tl.lds_fwd, stream, pr, tl);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess || tl.split0 == tiles) return e;
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  if (ev1 != nullptr && (e = hipEventRecord(ev1, stream)) != hipSuccess) return e;
+  if (tl.split0 == tiles) return hipSuccess;
   hipLaunchKernelGGL((forward_combine_kernel<V, DIFF, SPEC>), dim3((unsigned)(tiles - tl.split0)),
                      dim3(64), 0, stream, pr, tl);
   return hipGetLastError();
@@ -1128,25 +1133,25 @@ hipError_t launch_bwd(const Problem& pr, const Tiling& tl, hipStream_t stream) {
 }  // namespace
 
 hipError_t launch_forward(int variant, bool diff, bool spec, const Problem& pr, const Tiling& tl,
-                          hipStream_t stream) {
+                          hipStream_t stream, hipEvent_t ev0, hipEvent_t ev1) {
   switch (variant) {
-    case kDiffuse: return launch_fwd<kDiffuse, true, false>(pr, tl, stream);
+    case kDiffuse: return launch_fwd<kDiffuse, true, false>(pr, tl, stream, ev0, ev1);
     case kSpecular:
-      if (diff && spec) return launch_fwd<kSpecular, true, true>(pr, tl, stream);
-      if (diff) return launch_fwd<kSpecular, true, false>(pr, tl, stream);
-      if (spec) return launch_fwd<kSpecular, false, true>(pr, tl, stream);
-      return launch_fwd<kSpecular, false, false>(pr, tl, stream);
+      if (diff && spec) return launch_fwd<kSpecular, true, true>(pr, tl, stream, ev0, ev1);
+      if (diff) return launch_fwd<kSpecular, true, false>(pr, tl, stream, ev0, ev1);
+      if (spec) return launch_fwd<kSpecular, false, true>(pr, tl, stream, ev0, ev1);
+      return launch_fwd<kSpecular, false, false>(pr, tl, stream, ev0, ev1);
     case kClamp:
-      if (diff && spec) return launch_fwd<kClamp, true, true>(pr, tl, stream);
-      if (diff) return launch_fwd<kClamp, true, false>(pr, tl, stream);
-      if (spec) return launch_fwd<kClamp, false, true>(pr, tl, stream);
-      return launch_fwd<kClamp, false, false>(pr, tl, stream);
+      if (diff && spec) return launch_fwd<kClamp, true, true>(pr, tl, stream, ev0, ev1);
+      if (diff) return launch_fwd<kClamp, true, false>(pr, tl, stream, ev0, ev1);
+      if (spec) return launch_fwd<kClamp, false, true>(pr, tl, stream, ev0, ev1);
+      return launch_fwd<kClamp, false, false>(pr, tl, stream, ev0, ev1);
     case kV2:
-      if (spec) return launch_fwd<kV2, true, true>(pr, tl, stream);
-      return launch_fwd<kV2, true, false>(pr, tl, stream);
+      if (spec) return launch_fwd<kV2, true, true>(pr, tl, stream, ev0, ev1);
+      return launch_fwd<kV2, true, false>(pr, tl, stream, ev0, ev1);
     case kVis:
-      if (spec) return launch_fwd<kVis, true, true>(pr, tl, stream);
-      return launch_fwd<kVis, true, false>(pr, tl, stream);
+      if (spec) return launch_fwd<kVis, true, true>(pr, tl, stream, ev0, ev1);
+      return launch_fwd<kVis, true, false>(pr, tl, stream, ev0, ev1);
     default: return hipErrorInvalidValue;
   }
 }

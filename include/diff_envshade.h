@@ -458,6 +458,11 @@ DES_API int des_blur_envmap_forward(int32_t C, int32_t H, int32_t W, int32_t tar
  * then clears the hook.  Pass NULLs to clear it explicitly. */
 DES_API int des_set_backward_timing_events(void* start, void* stop);
 
+/* The same for the calling thread's next *_forward* call: `start` right
+ * before and `stop` right after its main shading kernel (the tail-slice
+ * combine, when there is one, runs after `stop`). */
+DES_API int des_set_forward_timing_events(void* start, void* stop);
+
 #ifdef __cplusplus
 }
 #endif

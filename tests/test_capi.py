@@ -199,6 +199,8 @@ def test_timing_events_need_both():
     L = _lib.load()
     assert L.des_set_backward_timing_events(ctypes.c_void_p(DUMMY), None) == -1
     assert L.des_set_backward_timing_events(None, None) == 0
+    assert L.des_set_forward_timing_events(ctypes.c_void_p(DUMMY), None) == -1
+    assert L.des_set_forward_timing_events(None, None) == 0
 
 
 # ---- mesh utilities ----------------------------------------------------------------
