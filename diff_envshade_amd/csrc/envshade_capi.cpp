@@ -93,11 +93,14 @@ des::Tiling tiling_for(int ntc, int T, bool backward) {
   tl.ntc = ntc;
   tl.cw = slots_per_wave * 32;
   tl.st = ntc * tl.cw;
-  // texel table, then the tile's visibility words (3 per point per slot);
-  // the cross-wave reduction at the end reuses the space from offset 0
+  // texel table, then the tile's visibility words (3 per point per slot),
+  // then the dark words (one bit per texel); the cross-wave reduction at the
+  // end reuses the space from offset 0
   const size_t table = size_t(des::kTexelBytes) * tl.st;
-  const size_t staged = table + size_t(tl.st / 32) * 3 * 64 * sizeof(uint32_t);
+  const size_t vis = size_t(tl.st / 32) * 3 * 64 * sizeof(uint32_t);
+  const size_t staged = table + vis + size_t((tl.st + 63) / 64) * sizeof(uint64_t);
   tl.vis_off = int(table);
+  tl.dark_off = int(table + vis);
   // (the tile's exchanged point fields also use the space from offset 0, first)
   const size_t xch = size_t(des::kPointXchSlots) * 64 * sizeof(float);
   tl.lds_fwd = std::max({staged, xch, size_t(ntc) * des::kForwardAcc * 64 * sizeof(float)});

@@ -52,6 +52,7 @@ struct Tiling {
   int cw;           // super-chunk texels per wave, st / ntc (multiple of 32)
   int st;           // texels per super-chunk = ntc * cw
   int vis_off;      // byte offset of the staged visibility words in dynamic LDS
+  int dark_off;     // byte offset of the super-chunk's dark words (one uint64 per 64 texels)
   size_t lds_fwd;   // dynamic LDS bytes, forward
   size_t lds_bwd;   // dynamic LDS bytes, backward
   // Backward tail slices: tiles [split0, tiles) are shaded as `slices`

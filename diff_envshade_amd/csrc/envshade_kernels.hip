@@ -62,52 +62,6 @@ __device__ __forceinline__ float reduce4(float q0, float q1, float q2, float q3)
   return row_allreduce(s);
 }
 
-// ---- per-texel table ---------------------------------------------------------
-
-// Texels [t0, t0+n) of the lat-long map into LDS slots [0, n); slots [n, n_alloc)
-// are zero so padded texels contribute exactly nothing.
-//   storage texel t: row r = t / W, column x = t % W, flipped row iy = H-1-r;
-//   theta = ((iy+.5)/H - .5) pi, phi = 2 pi (x+.5)/W, d = (cos th cos ph, sin th, cos th sin ph)
-//   dOmega = (sin(pi(iy+1)/H - pi/2) - sin(pi iy/H - pi/2)) 2pi/W
-//          = 2 sin(pi/(2H)) cos(theta) 2pi/W   (exact identity, no cancellation)
-//   l = M d with M row-major.
-// Texels [t0, t0+n) of the lat-long map into LDS entries [0, n); entries
-// [n, n_alloc) are zero so padded texels contribute exactly nothing.
-__device__ void build_table(const Problem& pr, int t0, int n, int n_alloc, Texel* tab) {
-  double M[9];
-#pragma unroll
-  for (int i = 0; i < 9; ++i) M[i] = (double)pr.rotmat[i];
-  const double kdw = solid_angle_scale(pr.H, pr.W);
-  const int T = pr.T;
-  for (int i = threadIdx.x; i < n_alloc; i += blockDim.x) {
-    if (i < n) {
-      const int t = t0 + i;
-      tab[i] = make_texel(texel_geometry(t, pr.H, pr.W, M, kdw), pr.envmap[t], pr.envmap[T + t],
-                          pr.envmap[2 * T + t]);
-    } else {
-      tab[i] = zero_texel();
-    }
-  }
-}
-
-// The forward's table (FwdTexel entries), same texels and zero padding.
-__device__ void build_fwd_table(const Problem& pr, int t0, int n, int n_alloc, FwdTexel* tab) {
-  double M[9];
-#pragma unroll
-  for (int i = 0; i < 9; ++i) M[i] = (double)pr.rotmat[i];
-  const double kdw = solid_angle_scale(pr.H, pr.W);
-  const int T = pr.T;
-  for (int i = threadIdx.x; i < n_alloc; i += blockDim.x) {
-    if (i < n) {
-      const int t = t0 + i;
-      tab[i] = make_fwd_texel(texel_geometry(t, pr.H, pr.W, M, kdw), pr.envmap[t],
-                              pr.envmap[T + t], pr.envmap[2 * T + t]);
-    } else {
-      tab[i] = zero_fwd_texel();
-    }
-  }
-}
-
 // ---- per-point state ---------------------------------------------------------
 
 template <int V, bool SPEC>
@@ -314,6 +268,11 @@ __device__ __forceinline__ VisWords staged_words(const uint32_t* vis, int sl, bo
 // 1e-4), far more than the fp32 rounding of either side: a texel it calls dark
 // has l.n < -1e-4 |l| |n| on every lane, which no lane's own fp32 cosine can
 // round up to a positive value, so skipping it changes no result.
+// The texels the cone test leaves lit with l.a <= 0 (a band about a great
+// circle, about 15% of a 16x32 map per sorted tile) are then tested
+// exactly: dark when the pair's own gate -- the same expression, bit for bit,
+// as the shading loop's -- is closed on every contributing lane.  Texels with
+// l.a > 0 light some lane (sum over lanes of l.n/|n| = |sum n/|n|| l.a > 0).
 struct Cone {
   float ax, ay, az;  // axis
   float sb;          // widened sin(beta); -4: every texel dark, 4: none
@@ -344,12 +303,19 @@ __device__ __forceinline__ float wave_min(float x) {
                fminf(lane_value(x, 32), lane_value(x, 48)));
 }
 
-// Cone of the normals of the lanes that can contribute (live, finite, non-zero).
+// Lanes that can contribute: live, with a finite non-zero normal.
+template <int V, bool SPEC>
+__device__ __forceinline__ bool cone_lane(const PointState<V, SPEC>& s) {
+  const float n2 = fmaf(s.nz, s.nz, fmaf(s.ny, s.ny, s.nx * s.nx));
+  return s.live != 0.f && n2 > 0.f && n2 < INFINITY;
+}
+
+// Cone of the normals of the lanes that can contribute.
 template <int V, bool SPEC>
 __device__ __forceinline__ Cone tile_cone(const PointState<V, SPEC>& s) {
   Cone c{0.f, 0.f, 0.f, 4.f};
   const float n2 = fmaf(s.nz, s.nz, fmaf(s.ny, s.ny, s.nx * s.nx));
-  const bool inc = s.live != 0.f && n2 > 0.f && n2 < INFINITY;
+  const bool inc = cone_lane(s);
   if (!__any(inc)) {
     c.sb = -4.f;
     return c;
@@ -370,20 +336,129 @@ __device__ __forceinline__ Cone tile_cone(const PointState<V, SPEC>& s) {
   return c;
 }
 
-// Dark bits of texels t0 .. t0+63 of the staged super-chunk (sn texels), bit
-// i for texel t0 + i; texels past sn read as dark.  Wave-uniform.
-template <class TX>
-__device__ __forceinline__ uint64_t dark_word(const TX* tab, const Cone& c, int t0, int sn,
-                                              int lane) {
-  const int t = t0 + lane;
-  bool dark = true;
-  if (t < sn) {
-    float lx, ly, lz;
-    texel_dir(tab[t], lx, ly, lz);
+// Dark bits of the 64 texels [base, base + 64) of a staged super-chunk, one
+// texel per lane (have: the lane's texel is one of the sn staged, with fp32
+// direction l): bit i set when texel base + i lights no lane of the tile, by
+// the cone test or, for the texels it leaves in doubt, by lit_any(i) (the
+// exact test, wave-uniform) returning false.  Texels past sn are dark.
+template <class LitAny>
+__device__ __forceinline__ uint64_t dark_bits(bool have, float lx, float ly, float lz,
+                                              const Cone& c, LitAny&& lit_any) {
+  bool dark = true, doubt = false;
+  if (have) {
     const float len = __builtin_amdgcn_sqrtf(fmaf(lz, lz, fmaf(ly, ly, lx * lx)));
-    dark = fmaf(lz, c.az, fmaf(ly, c.ay, lx * c.ax)) <= -c.sb * len;
+    const float la = fmaf(lz, c.az, fmaf(ly, c.ay, lx * c.ax));
+    dark = la <= -c.sb * len;
+    doubt = !dark && la <= 0.f;
   }
-  return __ballot(dark);
+  uint64_t dk = __ballot(dark);
+  uint64_t todo = __ballot(doubt);
+#pragma unroll 1
+  while (todo != 0ull) {
+    const int i = __builtin_ctzll(todo);
+    todo &= todo - 1ull;
+    if (!lit_any(i)) dk |= 1ull << i;
+  }
+  return dk;
+}
+
+// The dark words of a staged super-chunk: word k (texels 64k .. 64k+63) at
+// dmask[k], written by the wave that builds those table entries.  A cone
+// wider than a hemisphere darkens no texel: all words 0.
+__device__ __forceinline__ uint64_t read_dark_word(const uint64_t* dmask, int k) {
+  const uint64_t w = dmask[k];
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)w);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(w >> 32));
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// ---- per-texel table ------------------------------------------------------------
+
+// Texels [t0, t0+n) of the lat-long map into LDS entries [0, n); entries
+// [n, n_alloc) are zero so padded texels contribute exactly nothing.
+//   storage texel t: row r = t / W, column x = t % W, flipped row iy = H-1-r;
+//   theta = ((iy+.5)/H - .5) pi, phi = 2 pi (x+.5)/W, d = (cos th cos ph, sin th, cos th sin ph)
+//   dOmega = (sin(pi(iy+1)/H - pi/2) - sin(pi iy/H - pi/2)) 2pi/W
+//          = 2 sin(pi/(2H)) cos(theta) 2pi/W   (exact identity, no cancellation)
+//   l = M d with M row-major.
+// Wave w builds the 64-entry blocks w, w + ntc, ... and with each block writes
+// its dark word dmask[block] (dark_bits against the tile's lanes); every word
+// up to entry n_alloc is written.
+template <int V, bool SPEC>
+__device__ void build_table(const Problem& pr, const PointState<V, SPEC>& st, const Cone& cone,
+                            int t0, int n, int n_alloc, Texel* tab, uint64_t* dmask) {
+  double M[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) M[i] = (double)pr.rotmat[i];
+  const double kdw = solid_angle_scale(pr.H, pr.W);
+  const int T = pr.T;
+  const int lane = threadIdx.x & 63;
+  const bool wide = cone.sb >= 1.f;
+  const bool inc = cone_lane(st);
+  for (int base = (int)threadIdx.x - lane; base < n_alloc; base += (int)blockDim.x) {
+    const int i = base + lane;
+    Texel e = zero_texel();
+    if (i < n) {
+      const int t = t0 + i;
+      e = make_texel(texel_geometry(t, pr.H, pr.W, M, kdw), pr.envmap[t], pr.envmap[T + t],
+                     pr.envmap[2 * T + t]);
+    }
+    if (i < n_alloc) tab[i] = e;
+    uint64_t dk = 0ull;
+    if (!wide) {
+      dk = dark_bits(i < n, e.lx, e.ly, e.lz, cone, [&](int j) {
+        const float lx = lane_value(e.lx, j), ly = lane_value(e.ly, j), lz = lane_value(e.lz, j);
+        // the backward's gate: eval_lobe's l.n > 0, the same expression
+        return __ballot(inc && fmaf(lz, st.nz, fmaf(ly, st.ny, lx * st.nx)) > 0.f) != 0ull;
+      });
+    }
+    if (lane == 0) dmask[base >> 6] = dk;
+  }
+}
+
+__device__ __forceinline__ double lane_value(double x, int l) {
+  const int2 v = *reinterpret_cast<const int2*>(&x);
+  int2 r;
+  r.x = __builtin_amdgcn_readlane(v.x, l);
+  r.y = __builtin_amdgcn_readlane(v.y, l);
+  return *reinterpret_cast<const double*>(&r);
+}
+
+// The forward's table (FwdTexel entries), same texels, zero padding and dark
+// words; the exact test is the forward's gate, fwd_pair's FP64 l.n > 0.
+template <int V, bool SPEC>
+__device__ void build_fwd_table(const Problem& pr, const PointState<V, SPEC>& st,
+                                const Cone& cone, int t0, int n, int n_alloc, FwdTexel* tab,
+                                uint64_t* dmask) {
+  double M[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) M[i] = (double)pr.rotmat[i];
+  const double kdw = solid_angle_scale(pr.H, pr.W);
+  const int T = pr.T;
+  const int lane = threadIdx.x & 63;
+  const bool wide = cone.sb >= 1.f;
+  const bool inc = cone_lane(st);
+  for (int base = (int)threadIdx.x - lane; base < n_alloc; base += (int)blockDim.x) {
+    const int i = base + lane;
+    FwdTexel e = zero_fwd_texel();
+    if (i < n) {
+      const int t = t0 + i;
+      e = make_fwd_texel(texel_geometry(t, pr.H, pr.W, M, kdw), pr.envmap[t], pr.envmap[T + t],
+                         pr.envmap[2 * T + t]);
+    }
+    if (i < n_alloc) tab[i] = e;
+    uint64_t dk = 0ull;
+    if (!wide) {
+      float lx, ly, lz;
+      texel_dir(e, lx, ly, lz);
+      dk = dark_bits(i < n, lx, ly, lz, cone, [&](int j) {
+        const double dx = lane_value(e.dx, j), dy = lane_value(e.dy, j), dz = lane_value(e.dz, j);
+        // max((float)nl, 0) > 0 needs nl > 0
+        return __ballot(inc && fma(dz, st.nzd, fma(dy, st.nyd, dx * st.nxd)) > 0.0) != 0ull;
+      });
+    }
+    if (lane == 0) dmask[base >> 6] = dk;
+  }
 }
 
 // Texel groups.  A staged super-chunk of sn texels is cut into groups of 4
@@ -395,12 +470,10 @@ __device__ __forceinline__ uint64_t dark_word(const TX* tab, const Cone& c, int 
 // depends only on the tile's dark bits, so the sums are reproducible.
 // lit(tg, d4) / dark(tg) get the group's first texel and, for lit groups, its
 // 4 dark bits.
-template <class TX, class Lit, class Dark>
-__device__ __forceinline__ void for_groups(const TX* tab, const Cone& cone, int sn, int wv,
-                                           int ntc, int lane, Lit&& lit, Dark&& dark) {
+template <class Lit, class Dark>
+__device__ __forceinline__ void for_groups(const uint64_t* dmask, int sn, int wv, int ntc,
+                                           int lane, Lit&& lit, Dark&& dark) {
   const int ngt = (sn + 3) >> 2;
-  // a cone wider than a hemisphere darkens no texel: skip the dark bits
-  const bool wide = cone.sb >= 1.f;
   int c = 0;  // lit groups before group k0 (wave-uniform)
 #pragma unroll 1
   for (int k0 = 0; k0 < ngt; k0 += 16) {
@@ -408,7 +481,7 @@ __device__ __forceinline__ void for_groups(const TX* tab, const Cone& cone, int 
     // Which of these 16 groups this wave owns, one group per lane (lane i
     // looks at group k0 + i), before any shading: the shading loop below then
     // has one exit, and its accumulators need no copies at control-flow joins.
-    const uint64_t dk = wide ? 0ull : dark_word(tab, cone, 4 * k0, sn, lane);
+    const uint64_t dk = read_dark_word(dmask, k0 >> 4);
     const uint64_t all4 = dk & (dk >> 1) & (dk >> 2) & (dk >> 3);  // bit 4 i: group i dark
     const bool in = lane < kn;
     const bool gdark = in && ((all4 >> (4 * (lane & 15))) & 1ull) != 0ull;
@@ -446,10 +519,10 @@ __device__ __forceinline__ void for_groups(const TX* tab, const Cone& cone, int 
 // acc = {sum E vis max(l.n,0) per channel, sum E vis F G D / 4 per channel}.
 template <int V, bool DIFF, bool SPEC, bool SINGLE>
 __device__ __forceinline__ void forward_chunk(const PointState<V, SPEC>& st, const FwdTexel* tab,
-                                              const uint32_t* svis, const Cone& cone, int sn,
+                                              const uint32_t* svis, const uint64_t* dmask, int sn,
                                               int wv, int ntc, int lane, float (&acc)[6]) {
   constexpr bool MASK = (V != kVis) || SINGLE;
-  for_groups(tab, cone, sn, wv, ntc, lane, [&](int tg, uint32_t) {
+  for_groups(dmask, sn, wv, ntc, lane, [&](int tg, uint32_t) {
     const VisWords w = staged_words<V>(svis, tg >> 5, SINGLE, lane);
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
@@ -542,16 +615,17 @@ __device__ __forceinline__ void forward_tile(const Problem& pr, const Tiling& tl
 
   FwdTexel* tab = reinterpret_cast<FwdTexel*>(smem);
   uint32_t* svis = reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(smem) + tl.vis_off);
+  uint64_t* dmask = reinterpret_cast<uint64_t*>(reinterpret_cast<char*>(smem) + tl.dark_off);
   for (int s0 = t_begin; s0 < t_end; s0 += tl.st) {
     const int sn = min(tl.st, t_end - s0);
     __syncthreads();
-    build_fwd_table(pr, s0, sn, min(tl.st, (sn + 31) & ~31), tab);
+    build_fwd_table(pr, st, cone, s0, sn, min(tl.st, (sn + 31) & ~31), tab, dmask);
     stage_vis(pr, st, s0, sn, single, svis, wv, tl.ntc, lane);
     __syncthreads();
     if (single)
-      forward_chunk<V, DIFF, SPEC, true>(st, tab, svis, cone, sn, wv, tl.ntc, lane, acc);
+      forward_chunk<V, DIFF, SPEC, true>(st, tab, svis, dmask, sn, wv, tl.ntc, lane, acc);
     else
-      forward_chunk<V, DIFF, SPEC, false>(st, tab, svis, cone, sn, wv, tl.ntc, lane, acc);
+      forward_chunk<V, DIFF, SPEC, false>(st, tab, svis, dmask, sn, wv, tl.ntc, lane, acc);
   }
   // the per-lane weight the masked sums leave out: the barycentric sum of a
   // single-word visibility tile, or the live flag of the other variants
@@ -604,12 +678,12 @@ struct BwdSums {
 // 168 registers that keep three waves per SIMD.
 template <int V, bool DIFF, bool SPEC>
 __device__ __forceinline__ void backward_chunk(const PointState<V, SPEC>& st, const Texel* tab,
-                                               const uint32_t* svis, const Cone& cone, int s0,
+                                               const uint32_t* svis, const uint64_t* dmask, int s0,
                                                int sn, int wv, int ntc, int lane, bool single,
                                                float csum, const float* A, const float* B, int T,
                                                float* part, int row_texel, int ch_out,
                                                BwdSums& a) {
-  for_groups(tab, cone, sn, wv, ntc, lane, [&](int tg, uint32_t d4) {
+  for_groups(dmask, sn, wv, ntc, lane, [&](int tg, uint32_t d4) {
     const int t_out = s0 + tg + row_texel;
     const VisWords w = staged_words<V>(svis, tg >> 5, single, lane);
     const int b4 = tg & 31;
@@ -798,6 +872,7 @@ __device__ __forceinline__ void backward_tile(const Problem& pr, const Tiling& t
 
   Texel* tab = reinterpret_cast<Texel*>(smem);
   uint32_t* svis = reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(smem) + tl.vis_off);
+  uint64_t* dmask = reinterpret_cast<uint64_t*>(reinterpret_cast<char*>(smem) + tl.dark_off);
   float* part = pr.part + tile * 3 * pr.T;
   // lanes (lane & 15) < 3 of row k store channel (lane & 15) of texel kRowTexel[k]
   const int row = lane >> 4;
@@ -807,10 +882,10 @@ __device__ __forceinline__ void backward_tile(const Problem& pr, const Tiling& t
   for (int s0 = t_begin; s0 < t_end; s0 += tl.st) {
     const int sn = min(tl.st, t_end - s0);
     __syncthreads();
-    build_table(pr, s0, sn, min(tl.st, (sn + 31) & ~31), tab);
+    build_table(pr, st, cone, s0, sn, min(tl.st, (sn + 31) & ~31), tab, dmask);
     stage_vis(pr, st, s0, sn, single, svis, wv, tl.ntc, lane);
     __syncthreads();
-    backward_chunk<V, DIFF, SPEC>(st, tab, svis, cone, s0, sn, wv, tl.ntc, lane, single, csum, A,
+    backward_chunk<V, DIFF, SPEC>(st, tab, svis, dmask, s0, sn, wv, tl.ntc, lane, single, csum, A,
                                   B, pr.T, part, row_texel, ch_out, a);
   }
 

@@ -4,7 +4,7 @@
 // with no lit lane: every texel in the opposite hemisphere of all 64 normals
 // has l.n <= 0 on every lane and contributes exactly nothing, so the shading
 // kernels skip it.  With random point order a tile's normals cover the sphere
-// and no texel is ever dark; sorted by normal direction about 43% of the
+// and no texel is ever dark; sorted by normal direction about 45% of the
 // (tile, texel) pairs of a 16x32 map are (at least a third in every tile).
 //
 // Key: the 10-bit Hilbert index of the normal's cell on a 32 x 32 octahedral
