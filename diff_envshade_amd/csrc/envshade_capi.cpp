@@ -98,12 +98,17 @@ des::Tiling tiling_for(int ntc, int T, bool backward) {
   // end reuses the space from offset 0
   const size_t table = size_t(des::kTexelBytes) * tl.st;
   const size_t vis = size_t(tl.st / 32) * 3 * 64 * sizeof(uint32_t);
-  const size_t staged = table + vis + size_t((tl.st + 63) / 64) * sizeof(uint64_t);
+  const size_t dark = size_t((tl.st + 63) / 64) * sizeof(uint64_t);
+  const size_t staged = table + vis + dark;
   tl.vis_off = int(table);
   tl.dark_off = int(table + vis);
+  // the forward's per-wave lists of lit texels follow
+  tl.list_off = int(staged);
+  tl.list_cap = ((tl.st + ntc - 1) / ntc + 3) & ~3;
+  const size_t staged_fwd = staged + size_t(ntc) * tl.list_cap * sizeof(uint16_t);
   // (the tile's exchanged point fields also use the space from offset 0, first)
   const size_t xch = size_t(des::kPointXchSlots) * 64 * sizeof(float);
-  tl.lds_fwd = std::max({staged, xch, size_t(ntc) * des::kForwardAcc * 64 * sizeof(float)});
+  tl.lds_fwd = std::max({staged_fwd, xch, size_t(ntc) * des::kForwardAcc * 64 * sizeof(float)});
   tl.lds_bwd = std::max({staged, xch, size_t(ntc) * des::kBackwardAcc * 64 * sizeof(float)});
   tl.split0 = INT64_MAX;  // set per problem by choose_tiling (single-set launches only)
   tl.slices = 1;

@@ -53,6 +53,8 @@ struct Tiling {
   int st;           // texels per super-chunk = ntc * cw
   int vis_off;      // byte offset of the staged visibility words in dynamic LDS
   int dark_off;     // byte offset of the super-chunk's dark words (one uint64 per 64 texels)
+  int list_off;     // forward: byte offset of the waves' lit-texel lists (uint16 entries)
+  int list_cap;     // forward: entries per wave, ceil(st / ntc) rounded up to 4
   size_t lds_fwd;   // dynamic LDS bytes, forward
   size_t lds_bwd;   // dynamic LDS bytes, backward
   // Backward tail slices: tiles [split0, tiles) are shaded as `slices`

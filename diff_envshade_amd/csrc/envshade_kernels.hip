@@ -506,50 +506,95 @@ __device__ __forceinline__ void for_groups(const uint64_t* dmask, int sn, int wv
   }
 }
 
+// Lit texels of a staged super-chunk, for the forward (which writes nothing
+// per texel): the lit texels -- those that light some lane -- are dealt to
+// the ntc waves in turn (the j-th lit texel to wave j mod ntc).  Each wave
+// first lists its own texels, in order, in LDS (list: room for ceil(sn/ntc)
+// entries, 8-byte aligned, written and read by this wave only), then shades
+// them four at a time (lit4: the four lobes form one block the compiler
+// interleaves) and the last 0-3 one at a time (lit1), in loops without other
+// branches, so the accumulators need no copies at control-flow joins.  The
+// deal depends only on the tile's dark bits, so the sums are reproducible.
+template <class Lit4, class Lit1>
+__device__ __forceinline__ void for_lit_texels(const uint64_t* dmask, int sn, int wv, int ntc,
+                                               int lane, uint16_t* list, Lit4&& lit4,
+                                               Lit1&& lit1) {
+  const uint64_t below = (1ull << lane) - 1ull;
+  int c = 0;      // lit texels dealt before word k
+  int nown = 0;   // this wave's texels listed so far
+#pragma unroll 1
+  for (int k = 0; 64 * k < sn; ++k) {
+    const int left = sn - 64 * k;
+    uint64_t litm = ~read_dark_word(dmask, k);
+    if (left < 64) litm &= (1ull << left) - 1ull;  // only the staged texels
+    const bool lit = ((litm >> lane) & 1ull) != 0ull;
+    const int rank = c + (int)__builtin_popcountll(litm & below);
+    const uint64_t own = __ballot(lit && (rank & (ntc - 1)) == wv);
+    if ((own >> lane) & 1ull) list[nown + (int)__builtin_popcountll(own & below)] = (uint16_t)(64 * k + lane);
+    nown += (int)__builtin_popcountll(own);
+    c += (int)__builtin_popcountll(litm);
+  }
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");  // the list, written above, read below
+  const uint2* l4 = reinterpret_cast<const uint2*>(list);
+  int b = 0;
+#pragma unroll 1
+  for (; b + 4 <= nown; b += 4) {
+    const uint2 e = l4[b >> 2];
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)e.x);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)e.y);
+    lit4((int)(lo & 0xffffu), (int)(lo >> 16), (int)(hi & 0xffffu), (int)(hi >> 16));
+  }
+#pragma unroll 1
+  for (; b < nown; ++b) lit1(__builtin_amdgcn_readfirstlane((int)list[b]));
+}
+
 // ---- forward -----------------------------------------------------------------
 
-// One wave's share of the texel groups of one staged super-chunk (sn texels).
-// A group that lights some lane is shaded whole, without a branch per texel,
-// so the four pairs form one block the compiler can interleave; its dark
-// texels add exact zeros (see Cone).  SINGLE: the wave-uniform case of one
-// visibility word per slot, chosen once per super-chunk; the word's bit is
-// applied as a mask and the lane's weight (the sum of its barycentrics) by
-// the caller, once per sum.  Other variants' per-lane weight (live) is also
-// applied by the caller.
+// One wave's share of the lit texels of one staged super-chunk (sn texels),
+// four pairs per block.  SINGLE: the wave-uniform case of one visibility word
+// per slot, chosen once per super-chunk; the word's bit is applied as a mask
+// and the lane's weight (the sum of its barycentrics) by the caller, once per
+// sum.  Other variants' per-lane weight (live) is also applied by the caller.
 // acc = {sum E vis max(l.n,0) per channel, sum E vis F G D / 4 per channel}.
 template <int V, bool DIFF, bool SPEC, bool SINGLE>
 __device__ __forceinline__ void forward_chunk(const PointState<V, SPEC>& st, const FwdTexel* tab,
-                                              const uint32_t* svis, const uint64_t* dmask, int sn,
-                                              int wv, int ntc, int lane, float (&acc)[6]) {
+                                              const uint32_t* svis, const uint64_t* dmask,
+                                              uint16_t* list, int sn, int wv, int ntc, int lane,
+                                              float (&acc)[6]) {
   constexpr bool MASK = (V != kVis) || SINGLE;
-  for_groups(dmask, sn, wv, ntc, lane, [&](int tg, uint32_t) {
-    const VisWords w = staged_words<V>(svis, tg >> 5, SINGLE, lane);
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const FwdTexel& g = tab[tg + i];
-      const int b = (tg & 31) + i;
-      uint32_t mask = ~0u;
-      float vis = 1.f;
-      if (V == kVis) {
-        if (SINGLE)
-          mask = (uint32_t)__builtin_amdgcn_sbfe((int)w.w0, b, 1);  // 0 or ~0
-        else
-          vis = vis_weight<V, SPEC>(st, w, b, false, 0.f);
-      }
-      float x, y;
-      fwd_pair<SPEC, MASK>(st, g, mask, vis, x, y);
-      if (DIFF) {
-        acc[0] = fmaf(g.e0, x, acc[0]);
-        acc[1] = fmaf(g.e1, x, acc[1]);
-        acc[2] = fmaf(g.e2, x, acc[2]);
-      }
-      if (SPEC) {
-        acc[3] = fmaf(g.e0, y, acc[3]);
-        acc[4] = fmaf(g.e1, y, acc[4]);
-        acc[5] = fmaf(g.e2, y, acc[5]);
-      }
+  auto shade = [&](int t) {
+    const FwdTexel& g = tab[t];
+    uint32_t mask = ~0u;
+    float vis = 1.f;
+    if (V == kVis) {
+      const VisWords w = staged_words<V>(svis, t >> 5, SINGLE, lane);
+      if (SINGLE)
+        mask = (uint32_t)__builtin_amdgcn_sbfe((int)w.w0, t & 31, 1);  // 0 or ~0
+      else
+        vis = vis_weight<V, SPEC>(st, w, t & 31, false, 0.f);
     }
-  }, [](int) {});
+    float x, y;
+    fwd_pair<SPEC, MASK>(st, g, mask, vis, x, y);
+    if (DIFF) {
+      acc[0] = fmaf(g.e0, x, acc[0]);
+      acc[1] = fmaf(g.e1, x, acc[1]);
+      acc[2] = fmaf(g.e2, x, acc[2]);
+    }
+    if (SPEC) {
+      acc[3] = fmaf(g.e0, y, acc[3]);
+      acc[4] = fmaf(g.e1, y, acc[4]);
+      acc[5] = fmaf(g.e2, y, acc[5]);
+    }
+  };
+  for_lit_texels(
+      dmask, sn, wv, ntc, lane, list,
+      [&](int t0, int t1, int t2, int t3) {
+        shade(t0);
+        shade(t1);
+        shade(t2);
+        shade(t3);
+      },
+      [&](int t) { shade(t); });
 }
 
 // One 64-point tile of one problem; called by every wave of the workgroup.
@@ -616,6 +661,8 @@ __device__ __forceinline__ void forward_tile(const Problem& pr, const Tiling& tl
   FwdTexel* tab = reinterpret_cast<FwdTexel*>(smem);
   uint32_t* svis = reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(smem) + tl.vis_off);
   uint64_t* dmask = reinterpret_cast<uint64_t*>(reinterpret_cast<char*>(smem) + tl.dark_off);
+  uint16_t* list = reinterpret_cast<uint16_t*>(reinterpret_cast<char*>(smem) + tl.list_off) +
+                   wv * tl.list_cap;
   for (int s0 = t_begin; s0 < t_end; s0 += tl.st) {
     const int sn = min(tl.st, t_end - s0);
     __syncthreads();
@@ -623,9 +670,9 @@ __device__ __forceinline__ void forward_tile(const Problem& pr, const Tiling& tl
     stage_vis(pr, st, s0, sn, single, svis, wv, tl.ntc, lane);
     __syncthreads();
     if (single)
-      forward_chunk<V, DIFF, SPEC, true>(st, tab, svis, dmask, sn, wv, tl.ntc, lane, acc);
+      forward_chunk<V, DIFF, SPEC, true>(st, tab, svis, dmask, list, sn, wv, tl.ntc, lane, acc);
     else
-      forward_chunk<V, DIFF, SPEC, false>(st, tab, svis, dmask, sn, wv, tl.ntc, lane, acc);
+      forward_chunk<V, DIFF, SPEC, false>(st, tab, svis, dmask, list, sn, wv, tl.ntc, lane, acc);
   }
   // the per-lane weight the masked sums leave out: the barycentric sum of a
   // single-word visibility tile, or the live flag of the other variants
