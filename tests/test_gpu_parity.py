@@ -187,6 +187,48 @@ def test_tail_slices_change_results_by_rounding_only():
         assert relerr(x, y) <= 1e-6
 
 
+def test_forward_on_a_handed_over_order(point_order):
+    # A forward that takes a handed-over point order (des_reuse_point_order_once)
+    # instead of sorting computes the same bits as one that sorts, whatever its
+    # workspace held before (tail-slice sums included: 70 000 points have 70
+    # tail tiles); the results are within tolerance of the oracle.
+    import torch
+    from diff_envshade_amd import _lib
+    from diff_envshade_amd import _diff_envshade as C
+    if point_order != 0:
+        pytest.skip("the automatic policy sorts this problem; one run suffices")
+    d = problem(seed=9, n=70000, H=16, W=32, vis="identity", n_bright=3)
+    t = {k: torch.from_numpy(np.ascontiguousarray(v)).cuda() for k, v in d.items()}
+    P, H, W = 70000, 16, 32
+    n = _lib.forward_workspace_size(P, H, W)
+    assert n > 0
+
+    def fwd(ws, order=None):
+        sh = torch.empty((P, 3), device="cuda")
+        ds = torch.empty((P, 3), device="cuda")
+        _lib.call("des_bruteforce_specularvisibility_shader_forward2", P, 3, H, W,
+                  t["face_vertex_list"].shape[1], t["visibility"].shape[1],
+                  t["visibility"].shape[0], C._ptr(t["normal"]), C._ptr(t["view_dir"]),
+                  C._ptr(t["albedo"]), C._ptr(t["specular_albedo"]), C._ptr(t["roughness"]),
+                  C._ptr(t["envmap"]), C._ptr(t["light2obj_rotmat"]),
+                  C._ptr(t["face_vertex_list"]), C._ptr(t["nearest_triangle_id"]),
+                  C._ptr(t["barycentric_coord"]), C._ptr(t["visibility"]), 1, C._ptr(sh),
+                  C._ptr(ds), C._ptr(ws), n, C._stream(torch.device("cuda")), order=order)
+        torch.cuda.synchronize()
+        return sh.cpu().numpy(), ds.cpu().numpy()
+
+    ws_a = torch.full((n,), 0xFF, dtype=torch.uint8, device="cuda")
+    a = fwd(ws_a)
+    ws_b = torch.full((n,), 0x5A, dtype=torch.uint8, device="cuda")
+    b = fwd(ws_b, order=(ws_a.data_ptr(), P))  # no sort
+    c = fwd(ws_a)  # sorts again over what the first call left
+    for x, y, z in zip(a, b, c):
+        assert np.array_equal(x, y) and np.array_equal(x, z)
+    from _helpers import O
+    sh, ds = O.forward("vis", d, True, True)
+    assert relerr(a[0], sh) <= TOL and relerr(a[1], ds) <= TOL
+
+
 def _kat_cases():
     import json
     import os
