@@ -1,18 +1,22 @@
 // envshade_kernels.hip -- gfx950 kernels for the brute-force environment-light
 // shader (forward, fused backward, deterministic env-map-gradient reduce).
 //
-// Work decomposition (one launch per op):
+// Work decomposition:
 //   * a workgroup owns one tile of 64 points (lane = point) and NTC waves;
-//     wave w shades those 64 points against texel chunk w (CW texels, a whole
-//     number of 32-texel visibility slots) of the current super-chunk;
-//   * the per-texel table (rotated light direction, |l|^2, E_c * dOmega,
-//     dOmega) is computed in FP64 from closed forms and rounded once to fp32
-//     into LDS; every lane reads the same entry (LDS broadcast);
+//     per staged super-chunk of texels the waves share the tile's texel
+//     table and visibility words in LDS, with one dark bit per texel (the
+//     texels that light no lane of the tile); the lit texels (forward) or
+//     lit groups of 4 texels (backward) are dealt to the waves in turn;
+//   * the per-texel table (rotated light direction in FP64, E_c * dOmega) is
+//     computed from closed forms in FP64 and rounded once where it is fp32;
+//     every lane reads the same entry (LDS broadcast);
 //   * per-point sums are reduced over the NTC waves through LDS in fixed
 //     order; per-texel env-map-gradient sums are reduced over the 64 lanes
 //     with permlane32/permlane16 swaps + row DPP (4 texels per reduction) and
-//     stored as per-tile partials; two small kernels then sum the partials in
-//     FP64 in a fixed order (bitwise reproducible, no atomics).
+//     stored as per-tile partials, which one more launch sums in FP64 in a
+//     fixed order (bitwise reproducible, no atomics);
+//   * the tiles beyond those resident at once run as texel slices, their
+//     per-point sums combined after the launch.
 //
 // Semantics follow zjumsj/diff_envshade bruteforce_shader.cu ("bs"):
 //   texel geometry bs:124-147 with the LEFT_TOP_AS_ORIGIN row flip (config.h:11);
@@ -720,9 +724,12 @@ struct BwdSums {
 
 // One wave's share of the texel groups of one staged super-chunk at s0 (sn
 // texels): per-point sums into a, and per group the tile's env-map partials
-// of its 4 texels into part.  Unlike the forward, dark texels of a lit group
-// are skipped one by one: four backward lobes in one block need more than the
-// 168 registers that keep three waves per SIMD.
+// of its 4 texels into part.  The backward deals groups, not texels as the
+// forward does, because its env-map partials are reduced per group of 4
+// (dealing lit texels in batches of 4 was measured slower: each texel then
+// computes its own table and visibility addresses).  Dark texels of a lit
+// group are skipped one by one: four backward lobes in one block need more
+// than the 128 registers that keep four waves per SIMD.
 template <int V, bool DIFF, bool SPEC>
 __device__ __forceinline__ void backward_chunk(const PointState<V, SPEC>& st, const Texel* tab,
                                                const uint32_t* svis, const uint64_t* dmask, int s0,
