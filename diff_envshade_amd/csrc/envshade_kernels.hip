@@ -1053,7 +1053,7 @@ __device__ __forceinline__ int64_t next_item(unsigned long long* counter) {
 }
 
 template <int V, bool DIFF, bool SPEC>
-__global__ __launch_bounds__(512) void batch_forward_kernel(const BatchHeader* __restrict__ hdr,
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void batch_forward_kernel(const BatchHeader* __restrict__ hdr,
                                                              const BatchSet* __restrict__ sets,
                                                              unsigned long long* counter,
                                                              int32_t sig) {
@@ -1071,7 +1071,7 @@ __global__ __launch_bounds__(512) void batch_forward_kernel(const BatchHeader* _
 }
 
 template <int V, bool DIFF, bool SPEC>
-__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(3))) void batch_backward_kernel(
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void batch_backward_kernel(
     const BatchHeader* __restrict__ hdr, const BatchSet* __restrict__ sets,
     unsigned long long* counter, int32_t sig) {
   if (!hdr->has_backward || hdr->sig_bwd != sig) return;
