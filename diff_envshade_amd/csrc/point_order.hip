@@ -249,17 +249,17 @@ __device__ __forceinline__ void scatter_block(const SortJob& job, int blk) {
   }
   uint32_t tot = 0u, pre = 0u;
   if (fused) {
-    // key tid's count in every block: one 4 KB row per block, read whole by
-    // the workgroup (eight rows in flight)
+    // key tid's count in every block: one 2 KB row per block, read whole by
+    // the workgroup (32 rows in flight: the reads are the scatter's latency)
     const uint16_t* col = reinterpret_cast<const uint16_t*>(count) + tid;
     const int nblk = job.nblk;
     int j = 0;
-    for (; j + 8 <= nblk; j += 8) {
-      uint32_t c[8];
+    for (; j + 32 <= nblk; j += 32) {
+      uint32_t c[32];
 #pragma unroll
-      for (int i = 0; i < 8; ++i) c[i] = col[(int64_t)(j + i) * kKeys];  // widened
+      for (int i = 0; i < 32; ++i) c[i] = col[(int64_t)(j + i) * kKeys];  // widened
 #pragma unroll
-      for (int i = 0; i < 8; ++i) {
+      for (int i = 0; i < 32; ++i) {
         tot += c[i];
         pre += (j + i < blk) ? c[i] : 0u;
       }
