@@ -151,7 +151,9 @@ def main() -> int:
     vis = [t["face_vertex_list"], t["nearest_triangle_id"], t["barycentric_coord"],
            t["visibility"]]
 
-    def step(kev=None):
+    def step(kev=None, fev=None):
+        if fev is not None:
+            _lib.set_forward_timing_events(fev[0], fev[1])
         sh, ds = C.bruteforce_specularvisibility_shader_forward2(*mat, *vis, True)
         if kev is not None:
             _lib.set_backward_timing_events(kev[0], kev[1])
@@ -176,43 +178,38 @@ def main() -> int:
     t1 = time.perf_counter()
     ms_per_step = (t1 - t0) * 1e3 / K
 
-    # latency and the backward kernel's duration: a second pass of K steps
-    # with HIP events around each step and around the backward launch
+    # latency and the main kernels' durations: a second pass of K steps with
+    # HIP events around each step and around the forward and backward
+    # launches (the library records the latter on the launch stream)
     ev = lambda: torch.cuda.Event(enable_timing=True)  # noqa: E731
     step_ev = [(ev(), ev()) for _ in range(K)]
     kern_ev = [(ev(), ev()) for _ in range(K)]
-    for a, b in kern_ev:  # torch creates the HIP events on first record
+    fkern_ev = [(ev(), ev()) for _ in range(K)]
+    for a, b in kern_ev + fkern_ev:  # torch creates the HIP events on first record
         a.record()
         b.record()
     torch.cuda.synchronize()
     t2 = time.perf_counter()
     for i in range(K):
         step_ev[i][0].record()
-        out = step(kern_ev[i])
+        out = step(kern_ev[i], fkern_ev[i])
         step_ev[i][1].record()
     torch.cuda.synchronize()
     ms_per_step_events = (time.perf_counter() - t2) * 1e3 / K
     step_ms = [a.elapsed_time(b) for a, b in step_ev]
     bwd_ms = [a.elapsed_time(b) for a, b in kern_ev]
     bwd_avg_ms = sum(bwd_ms) / len(bwd_ms)
+    fwd_avg_ms = sum(a.elapsed_time(b) for a, b in fkern_ev) / K
 
-    # the forward call (sort, main kernel, tail combine) and its main kernel,
-    # both with HIP events on the call's stream
+    # the whole forward call (sort, main kernel, tail combine), HIP events
     nf = min(K, 100)
     call_ev = [(ev(), ev()) for _ in range(nf)]
-    fkern_ev = [(ev(), ev()) for _ in range(nf)]
-    for a, b in fkern_ev:
+    for a, b in call_ev:
         a.record()
-        b.record()
-    torch.cuda.synchronize()
-    for (a, b), (ka, kb) in zip(call_ev, fkern_ev):
-        a.record()
-        _lib.set_forward_timing_events(ka, kb)
         C.bruteforce_specularvisibility_shader_forward2(*mat, *vis, True)
         b.record()
     torch.cuda.synchronize()
     fwd_call_ms = sum(a.elapsed_time(b) for a, b in call_ev) / nf
-    fwd_avg_ms = sum(a.elapsed_time(b) for a, b in fkern_ev) / nf
     print(f"[bench] timed: {ms_per_step:.4f} ms/step, backward kernel {bwd_avg_ms:.4f} ms, "
           f"forward kernel {fwd_avg_ms:.4f} ms (forward call {fwd_call_ms:.4f} ms)",
           file=sys.stderr, flush=True)
@@ -247,8 +244,9 @@ def main() -> int:
         "kernels_ms": {"forward_main": fwd_avg_ms, "backward_main": bwd_avg_ms,
                        "forward_call": fwd_call_ms,
                        "note": "main shading kernels from HIP events the library records around "
-                               "their launch; forward_call: the whole forward call (sort, "
-                               "main kernel, tail-slice combine)"},
+                               "their launch, in the instrumented pass of K steps; "
+                               "forward_call: the whole forward call (sort, main kernel, "
+                               "tail-slice combine), forward calls alone"},
         "roofline": {
             "bound": "mfma",
             "pipe": "fp32 vector ALU (no MFMA on this path)",
@@ -623,7 +621,9 @@ def bench_mesh_vis(args):
     vis = [t["face_vertex_list"], t["nearest_triangle_id"], t["barycentric_coord"],
            t["visibility"]]
 
-    def step(kev=None):
+    def step(kev=None, fev=None):
+        if fev is not None:
+            _lib.set_forward_timing_events(fev[0], fev[1])
         sh, ds = C.bruteforce_specularvisibility_shader_forward2(*mat, *vis, True)
         if kev is not None:
             _lib.set_backward_timing_events(kev[0], kev[1])
