@@ -288,10 +288,15 @@ class GpuExecutor:
     Results: `on_result(request, outputs)` receives each request's output
     tensors when its batch retires; without it they are left in
     request.result.  The executor drops its own references then, so what the
-    caller does not keep is freed."""
+    caller does not keep is freed.
 
-    def __init__(self, variant: str = "vis", slots: int = 3, ntc_forward: int = 8,
-                 ntc_backward: int = 8, workspace_bytes: int = 2 << 30, graphs: bool = True,
+    Launch shape: 4 waves per 64-point tile in both directions by default
+    (four tiles per CU, the batched launches' best); 8 waves per tile finish a
+    small batch's few tiles sooner but served 16% fewer requests per second
+    on the mixed stream at the same p99."""
+
+    def __init__(self, variant: str = "vis", slots: int = 3, ntc_forward: int = 4,
+                 ntc_backward: int = 4, workspace_bytes: int = 2 << 30, graphs: bool = True,
                  on_result: Optional[Callable] = None, device=None):
         import torch
 
@@ -299,6 +304,8 @@ class GpuExecutor:
         self.torch = torch
         self.device = torch.device(device) if device is not None else torch.device("cuda", 0)
         self.slots = slots
+        self.ntc_forward = ntc_forward  # the launch shape: results are bitwise those of a
+        self.ntc_backward = ntc_backward  # lone ShadeBatch with these
         self.graphs = graphs
         self.on_result = on_result
         self._free = list(range(slots))

@@ -60,9 +60,9 @@ def _serve(pool, wl, offsets, budget, slots=3, workspace_bytes=2 << 30):
     return reqs, got, ex, sch, peak
 
 
-def _check_alone(pool, wl, offsets, reqs, got):
+def _check_alone(pool, wl, offsets, reqs, got, ex):
     from diff_envshade_amd.batch import ShadeBatch
-    b = ShadeBatch("vis", ntc_forward=8, ntc_backward=8)
+    b = ShadeBatch("vis", ntc_forward=ex.ntc_forward, ntc_backward=ex.ntc_backward)
     for r in reqs:
         a = _alone(b, pool.request_payload(r, int(wl.env_ids[r.rid]), int(offsets[r.rid])),
                    r.backward)
@@ -134,7 +134,7 @@ def test_serving_config_alone_vs_mix_oracle_overlap_and_memory():
         _check_oracle(pool, wl, offsets, r, got[r.rid])
 
     # every request alone gives bitwise the result it got in the mix
-    _check_alone(pool, wl, offsets, reqs, got)
+    _check_alone(pool, wl, offsets, reqs, got, ex)
 
 
 def test_tight_budget_bounds_what_the_server_allocates():
@@ -155,4 +155,4 @@ def test_tight_budget_bounds_what_the_server_allocates():
     assert sch.peak_in_use > (budget - ex.fixed_bytes) // 2  # the budget was what bound
     assert peak <= sch.peak_in_use, (peak, sch.peak_in_use)
     assert ex.fixed_bytes + sch.peak_in_use <= budget
-    _check_alone(pool, wl, offsets, reqs, got)
+    _check_alone(pool, wl, offsets, reqs, got, ex)
