@@ -555,7 +555,9 @@ __device__ __forceinline__ void for_lit_texels(const uint64_t* dmask, int sn, in
 // ---- forward -----------------------------------------------------------------
 
 // One wave's share of the lit texels of one staged super-chunk (sn texels),
-// four pairs per block.  SINGLE: the wave-uniform case of one visibility word
+// four pairs per block; when the tile's cone is wider than a hemisphere (no
+// texel can be dark: unsorted points) the groups of 4 texels are dealt in a
+// fixed stride instead, without the lists.  SINGLE: the wave-uniform case of one visibility word
 // per slot, chosen once per super-chunk; the word's bit is applied as a mask
 // and the lane's weight (the sum of its barycentrics) by the caller, once per
 // sum.  Other variants' per-lane weight (live) is also applied by the caller.
@@ -563,8 +565,8 @@ __device__ __forceinline__ void for_lit_texels(const uint64_t* dmask, int sn, in
 template <int V, bool DIFF, bool SPEC, bool SINGLE>
 __device__ __forceinline__ void forward_chunk(const PointState<V, SPEC>& st, const FwdTexel* tab,
                                               const uint32_t* svis, const uint64_t* dmask,
-                                              uint16_t* list, int sn, int wv, int ntc, int lane,
-                                              float (&acc)[6]) {
+                                              uint16_t* list, bool wide, int sn, int wv, int ntc,
+                                              int lane, float (&acc)[6]) {
   constexpr bool MASK = (V != kVis) || SINGLE;
   auto shade = [&](int t) {
     const FwdTexel& g = tab[t];
@@ -590,6 +592,18 @@ __device__ __forceinline__ void forward_chunk(const PointState<V, SPEC>& st, con
       acc[5] = fmaf(g.e2, y, acc[5]);
     }
   };
+  if (wide) {
+    // no texel is dark (unsorted points): wave w takes groups of 4 texels
+    // w, w + ntc, ...; texels past sn are zero entries of the table
+#pragma unroll 1
+    for (int tg = 4 * wv; tg < sn; tg += 4 * ntc) {
+      shade(tg);
+      shade(tg + 1);
+      shade(tg + 2);
+      shade(tg + 3);
+    }
+    return;
+  }
   for_lit_texels(
       dmask, sn, wv, ntc, lane, list,
       [&](int t0, int t1, int t2, int t3) {
@@ -673,10 +687,13 @@ __device__ __forceinline__ void forward_tile(const Problem& pr, const Tiling& tl
     build_fwd_table(pr, st, cone, s0, sn, min(tl.st, (sn + 31) & ~31), tab, dmask);
     stage_vis(pr, st, s0, sn, single, svis, wv, tl.ntc, lane);
     __syncthreads();
+    const bool wide = cone.sb >= 1.f;  // a cone wider than a hemisphere: no dark texel
     if (single)
-      forward_chunk<V, DIFF, SPEC, true>(st, tab, svis, dmask, list, sn, wv, tl.ntc, lane, acc);
+      forward_chunk<V, DIFF, SPEC, true>(st, tab, svis, dmask, list, wide, sn, wv, tl.ntc, lane,
+                                         acc);
     else
-      forward_chunk<V, DIFF, SPEC, false>(st, tab, svis, dmask, list, sn, wv, tl.ntc, lane, acc);
+      forward_chunk<V, DIFF, SPEC, false>(st, tab, svis, dmask, list, wide, sn, wv, tl.ntc, lane,
+                                          acc);
   }
   // the per-lane weight the masked sums leave out: the barycentric sum of a
   // single-word visibility tile, or the live flag of the other variants
