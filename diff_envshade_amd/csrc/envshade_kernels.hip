@@ -212,15 +212,14 @@ struct VisWords {
   uint32_t w0, w1, w2;
 };
 
-// vis weight for bit b: sum_j c_j * bit_j (bs:2186-2190).  `single` is the
-// wave-uniform case where all three vertices of every lane coincide (one word
-// per slot, weight c0+c1+c2).
+// vis weight for bit b: sum_j c_j * bit_j (bs:2186-2190); the live flag of the
+// other variants.  The wave-uniform case where all three vertices of every
+// lane coincide (one word per slot, weight bit * (c0+c1+c2)) is taken by the
+// shading loops as a bit mask instead.
 template <int V, bool SPEC>
-__device__ __forceinline__ float vis_weight(const PointState<V, SPEC>& s, const VisWords& w, int b,
-                                            bool single, float csum) {
+__device__ __forceinline__ float vis_weight(const PointState<V, SPEC>& s, const VisWords& w, int b) {
   if (V != kVis) return s.live;
   const float b0 = (float)__builtin_amdgcn_ubfe(w.w0, b, 1);
-  if (single) return b0 * csum;
   const float b1 = (float)__builtin_amdgcn_ubfe(w.w1, b, 1);
   const float b2 = (float)__builtin_amdgcn_ubfe(w.w2, b, 1);
   return fmaf(b2, s.c2, fmaf(b1, s.c1, b0 * s.c0));
@@ -577,7 +576,7 @@ __device__ __forceinline__ void forward_chunk(const PointState<V, SPEC>& st, con
       if (SINGLE)
         mask = (uint32_t)__builtin_amdgcn_sbfe((int)w.w0, t & 31, 1);  // 0 or ~0
       else
-        vis = vis_weight<V, SPEC>(st, w, t & 31, false, 0.f);
+        vis = vis_weight<V, SPEC>(st, w, t & 31);
     }
     float x, y;
     fwd_pair<SPEC, MASK>(st, g, mask, vis, x, y);
@@ -747,16 +746,23 @@ struct BwdSums {
 // computes its own table and visibility addresses).  Dark texels of a lit
 // group are skipped one by one: four backward lobes in one block need more
 // than the 128 registers that keep four waves per SIMD.
-template <int V, bool DIFF, bool SPEC>
+//
+// The per-point factor ib/4 of F G D / 4 is left out of the pair's lobe (Gq =
+// G1(l) = nl ia, y = vis F G1(l) D) and rides on B (B_c ib/4); the sums of y
+// and of grad_a G1(l) are scaled by it once per tile (backward_points).  SINGLE
+// is the visibility variant's one-word tile: the pair's bit masks max(l.n, 0)
+// and the per-lane weight c0+c1+c2 rides on A and B (and on the tile's sums of
+// x and y), so the pair needs no float weight.
+template <int V, bool DIFF, bool SPEC, bool SINGLE>
 __device__ __forceinline__ void backward_chunk(const PointState<V, SPEC>& st, const Texel* tab,
                                                const uint32_t* svis, const uint64_t* dmask, int s0,
-                                               int sn, int wv, int ntc, int lane, bool single,
-                                               float csum, const float* A, const float* B, int T,
-                                               float* part, int row_texel, int ch_out,
-                                               BwdSums& a) {
+                                               int sn, int wv, int ntc, int lane, const float* A,
+                                               const float* B, int T, float* part, int row_texel,
+                                               int ch_out, BwdSums& a) {
+  static_assert(!SINGLE || V == kVis, "one-word tiles are the visibility variant's");
   for_groups(dmask, sn, wv, ntc, lane, [&](int tg, uint32_t d4) {
     const int t_out = s0 + tg + row_texel;
-    const VisWords w = staged_words<V>(svis, tg >> 5, single, lane);
+    const VisWords w = staged_words<V>(svis, tg >> 5, SINGLE, lane);
     const int b4 = tg & 31;
     {
       float q[4][3];
@@ -765,32 +771,38 @@ __device__ __forceinline__ void backward_chunk(const PointState<V, SPEC>& st, co
         q[i][0] = q[i][1] = q[i][2] = 0.f;
         if ((d4 >> i) & 1u) continue;
         const Texel& g = tab[tg + i];
-        const float vis = vis_weight<V, SPEC>(st, w, b4 + i, single, csum);
-        const Lobe lb = eval_lobe<SPEC>(st, g);
+        // the pair's weight: a bit mask (SINGLE) or a float (vis)
+        const uint32_t m = SINGLE ? (uint32_t)__builtin_amdgcn_sbfe((int)w.w0, b4 + i, 1) : ~0u;
+        const float vis = SINGLE ? 1.f : vis_weight<V, SPEC>(st, w, b4 + i);
+        const Lobe lb = eval_lobe<SPEC, false>(st, g, m);
         float coef_n = 0.f;
         float x = 0.f, y = 0.f;
         if (DIFF) {
-          x = lb.nl * vis;
+          x = SINGLE ? lb.nl : lb.nl * vis;
           a.ds0 = fmaf(g.e0, x, a.ds0);
           a.ds1 = fmaf(g.e1, x, a.ds1);
           a.ds2 = fmaf(g.e2, x, a.ds2);
           // gdot = vis [l.n > 0] sum_c A_c E'_c   (bs:2426-2438)
           const float AE = fmaf(A[2], g.e2, fmaf(A[1], g.e1, A[0] * g.e0));
-          coef_n = (lb.cs > 0.f) ? AE * vis : 0.f;
+          if (SINGLE)
+            coef_n = (lb.nl > 0.f) ? AE : 0.f;
+          else
+            coef_n = (lb.cs > 0.f) ? AE * vis : 0.f;
         }
         if (SPEC) {
-          y = lb.tmp * vis;
+          y = SINGLE ? lb.tmp : lb.tmp * vis;
           a.sp0 = fmaf(g.e0, y, a.sp0);
           a.sp1 = fmaf(g.e1, y, a.sp1);
           a.sp2 = fmaf(g.e2, y, a.sp2);
           const float BE = fmaf(B[2], g.e2, fmaf(B[1], g.e1, B[0] * g.e0));
-          const float gt = lb.spec_on ? BE * vis : 0.f;  // grad of tmp = F G D / 4
+          // grad of tmp = F G D / 4, times ib/4 (carried by B)
+          const float gt = lb.spec_on ? (SINGLE ? BE : BE * vis) : 0.f;
           const float PF = gt * lb.F;
           const float gF = gt * (lb.Gq * lb.D);
           const float gD = PF * lb.Gq;
           // G1 VJP (bs:173-201), with grad_a = gG ib = (gt F D / 4) ib and
           // grad_b = -gG G ib = -4 grad_a G/4; l.n > 0 holds wherever gt != 0
-          const float grad_a = (PF * lb.D) * st.ib4;
+          const float grad_a = PF * lb.D;
           const float w = grad_a * (lb.ia * lb.ia);
           const float gdl = w * st.k;
           a.gk = fmaf(w, fmaf(lb.nl, lb.nl, -lb.nl), a.gk);
@@ -897,6 +909,85 @@ __device__ __forceinline__ void write_backward_outputs(const Problem& pr,
 //   terms of the NDF and normalize VJPs are accumulated along l + v and along
 //   hb x (n x hb); the G1 parts along n and v are per-point constants times a
 //   scalar sum (Sb), applied once at the end.
+// The rest of a tile once its points are read (backward_tile): the texel
+// loop, the reduction over the tile's waves and the outputs.  SINGLE is the
+// visibility variant's one-word tile.
+template <int V, bool DIFF, bool SPEC, bool SINGLE>
+__device__ __forceinline__ void backward_points(const Problem& pr, const Tiling& tl, int64_t tile,
+                                                const PointState<V, SPEC>& st, int64_t p,
+                                                bool valid, const float* gs, float* A, float* B,
+                                                int t_begin, int t_end, int64_t slice_item,
+                                                int lane, int wv) {
+  extern __shared__ float4 smem[];
+  // the pair weight's per-lane factor: c0+c1+c2 on a one-word tile
+  const float cw = SINGLE ? st.c0 + st.c1 + st.c2 : 1.f;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    A[c] *= cw;
+    B[c] *= cw * st.ib4;
+  }
+
+  const Cone cone = tile_cone(st);
+
+  BwdSums a{};
+
+  Texel* tab = reinterpret_cast<Texel*>(smem);
+  uint32_t* svis = reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(smem) + tl.vis_off);
+  uint64_t* dmask = reinterpret_cast<uint64_t*>(reinterpret_cast<char*>(smem) + tl.dark_off);
+  float* part = pr.part + tile * 3 * pr.T;
+  // lanes (lane & 15) < 3 of row k store channel (lane & 15) of texel kRowTexel[k]
+  const int row = lane >> 4;
+  const int row_texel = (row == 1) ? 2 : ((row == 2) ? 1 : row);
+  const int ch_out = lane & 15;
+
+  for (int s0 = t_begin; s0 < t_end; s0 += tl.st) {
+    const int sn = min(tl.st, t_end - s0);
+    __syncthreads();
+    build_table(pr, st, cone, s0, sn, min(tl.st, (sn + 31) & ~31), tab, dmask);
+    stage_vis(pr, st, s0, sn, SINGLE, svis, wv, tl.ntc, lane);
+    __syncthreads();
+    backward_chunk<V, DIFF, SPEC, SINGLE>(st, tab, svis, dmask, s0, sn, wv, tl.ntc, lane, A, B,
+                                          pr.T, part, row_texel, ch_out, a);
+  }
+  // the per-point factors the pair sums leave out: c0+c1+c2 of a one-word
+  // tile on the sums of x and y, ib/4 on those of y and grad_a G1(l)
+  a.ds0 *= cw;
+  a.ds1 *= cw;
+  a.ds2 *= cw;
+  a.sp0 *= cw * st.ib4;
+  a.sp1 *= cw * st.ib4;
+  a.sp2 *= cw * st.ib4;
+  a.Sq *= st.ib4;
+
+  // fixed-order reduction of the per-point sums over the NTC texel chunks
+  constexpr int NACC = kBackwardAcc;
+  float acc[NACC] = {a.ds0, a.ds1, a.ds2, a.sp0, a.sp1, a.sp2, a.gnx, a.gny,
+                     a.gnz, a.gvx, a.gvy, a.gvz, a.Sq,  a.ga2a, a.ga2b, a.gk};
+  if (tl.ntc > 1) {
+    __syncthreads();
+    float* red = reinterpret_cast<float*>(smem);
+#pragma unroll
+    for (int j = 0; j < NACC; ++j) red[(wv * NACC + j) * 64 + lane] = acc[j];
+    __syncthreads();
+    if (wv != 0) return;
+#pragma unroll
+    for (int j = 0; j < NACC; ++j) {
+      float sum = red[j * 64 + lane];
+      for (int w = 1; w < tl.ntc; ++w) sum += red[(w * NACC + j) * 64 + lane];
+      acc[j] = sum;
+    }
+  }
+  if (slice_item >= 0) {
+    // a tail slice: its per-point sums, combined after the launch
+    float* out = pr.spart + (int64_t)slice_item * NACC * 64;
+#pragma unroll
+    for (int j = 0; j < NACC; ++j) out[j * 64 + lane] = acc[j];
+    return;
+  }
+  if (!valid) return;
+  write_backward_outputs<V, DIFF, SPEC>(pr, st, p, gs, acc);
+}
+
 // Texels [t_begin, t_end) of one 64-point tile (t_begin a multiple of 32);
 // slice_item >= 0 marks a tail slice, whose per-point sums go to
 // pr.spart[slice_item] instead of the outputs.
@@ -930,63 +1021,15 @@ __device__ __forceinline__ void backward_tile(const Problem& pr, const Tiling& t
       if (SPEC) B[c] = gs[c] * xch[(22 + c) * 64 + lane];
     }
   }
-  bool single = true;
-  float csum = 0.f;
-  if (V == kVis) {
-    single = __all(st.vert0 == st.vert1 && st.vert1 == st.vert2);
-    csum = st.c0 + st.c1 + st.c2;
-  }
-
-  const Cone cone = tile_cone(st);
-
-  BwdSums a{};
-
-  Texel* tab = reinterpret_cast<Texel*>(smem);
-  uint32_t* svis = reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(smem) + tl.vis_off);
-  uint64_t* dmask = reinterpret_cast<uint64_t*>(reinterpret_cast<char*>(smem) + tl.dark_off);
-  float* part = pr.part + tile * 3 * pr.T;
-  // lanes (lane & 15) < 3 of row k store channel (lane & 15) of texel kRowTexel[k]
-  const int row = lane >> 4;
-  const int row_texel = (row == 1) ? 2 : ((row == 2) ? 1 : row);
-  const int ch_out = lane & 15;
-
-  for (int s0 = t_begin; s0 < t_end; s0 += tl.st) {
-    const int sn = min(tl.st, t_end - s0);
-    __syncthreads();
-    build_table(pr, st, cone, s0, sn, min(tl.st, (sn + 31) & ~31), tab, dmask);
-    stage_vis(pr, st, s0, sn, single, svis, wv, tl.ntc, lane);
-    __syncthreads();
-    backward_chunk<V, DIFF, SPEC>(st, tab, svis, dmask, s0, sn, wv, tl.ntc, lane, single, csum, A,
-                                  B, pr.T, part, row_texel, ch_out, a);
-  }
-
-  // fixed-order reduction of the per-point sums over the NTC texel chunks
-  constexpr int NACC = kBackwardAcc;
-  float acc[NACC] = {a.ds0, a.ds1, a.ds2, a.sp0, a.sp1, a.sp2, a.gnx, a.gny,
-                     a.gnz, a.gvx, a.gvy, a.gvz, a.Sq,  a.ga2a, a.ga2b, a.gk};
-  if (tl.ntc > 1) {
-    __syncthreads();
-    float* red = reinterpret_cast<float*>(smem);
-#pragma unroll
-    for (int j = 0; j < NACC; ++j) red[(wv * NACC + j) * 64 + lane] = acc[j];
-    __syncthreads();
-    if (wv != 0) return;
-#pragma unroll
-    for (int j = 0; j < NACC; ++j) {
-      float sum = red[j * 64 + lane];
-      for (int w = 1; w < tl.ntc; ++w) sum += red[(w * NACC + j) * 64 + lane];
-      acc[j] = sum;
-    }
-  }
-  if (slice_item >= 0) {
-    // a tail slice: its per-point sums, combined after the launch
-    float* out = pr.spart + (int64_t)slice_item * NACC * 64;
-#pragma unroll
-    for (int j = 0; j < NACC; ++j) out[j * 64 + lane] = acc[j];
-    return;
-  }
-  if (!valid) return;
-  write_backward_outputs<V, DIFF, SPEC>(pr, st, p, gs, acc);
+  // the visibility variant's one-word tiles take their own code path, so that
+  // its registers are allocated apart from the general path's
+  const bool single = (V == kVis) && __all(st.vert0 == st.vert1 && st.vert1 == st.vert2);
+  if (single)
+    backward_points<V, DIFF, SPEC, V == kVis>(pr, tl, tile, st, p, valid, gs, A, B, t_begin, t_end,
+                                              slice_item, lane, wv);
+  else
+    backward_points<V, DIFF, SPEC, false>(pr, tl, tile, st, p, valid, gs, A, B, t_begin, t_end,
+                                          slice_item, lane, wv);
 }
 
 // ---- single-problem launches: one workgroup per tile ---------------------------

@@ -136,13 +136,13 @@ struct Lobe {
   float nh;    // n.h
   float om4;   // (1 - sat(ch))^4
   float F;
-  float Gq;    // G / 4, G = nl ia ib
+  float Gq;    // G / 4, G = nl ia ib (without IB: nl ia, the per-point ib / 4 left out)
   float D;     // alpha^2 / max(pi tD^2, thres)
   float ia;    // 1/(nl(1-k)+k)
   float tD;    // (n.h)^2 (alpha^2-1) + 1
   float tD2;   // tD^2
   float itD2;  // 1/max(tD^2, thres/pi) = pi / max(pi tD^2, thres)
-  float tmp;   // F G D / 4, or 0 where the reference skips the lobe (tmp <= 0 or NaN)
+  float tmp;   // F Gq D, or 0 where the reference skips the lobe (tmp <= 0 or NaN)
   bool spec_on;
 };
 
@@ -160,12 +160,15 @@ struct Lobe {
 // n - (n.h) h = (hb x (n x hb)) / |hb|^2 is formed the same way, as
 // n |hb|^2 - hb (n.hb) in FP64 (size ~ sin(theta_h), no 1/sin cancellation).
 // The constant factors 1/4 (of F G D / 4) and 1/pi (of D) ride on the
-// per-point constants ib4 = ib/4 and a2pi = alpha^2/pi.
-template <bool SPEC, class S>
-__device__ __forceinline__ Lobe eval_lobe(const S& s, const Texel& g) {
+// per-point constants ib4 = ib/4 and a2pi = alpha^2/pi.  Without IB, Gq and
+// tmp leave out ib4, which every term of a pair carries, so that a caller can
+// apply it to its sums once.  `mask` (0 or ~0, a 0/1 visibility bit) is
+// applied to max(l.n, 0) as a bit mask: G and with it tmp are 0 where it is 0.
+template <bool SPEC, bool IB = true, class S>
+__device__ __forceinline__ Lobe eval_lobe(const S& s, const Texel& g, uint32_t mask = ~0u) {
   Lobe o;
   o.cs = fmaf(g.lz, s.nz, fmaf(g.ly, s.ny, g.lx * s.nx));
-  o.nl = fmaxf(o.cs, 0.f);
+  o.nl = __uint_as_float(__float_as_uint(fmaxf(o.cs, 0.f)) & mask);
   if (SPEC) {
     // h = (l+v)/|l+v| (bs:2218-2219)
     const double hx = g.dx + s.vxd, hy = g.dy + s.vyd, hz = g.dz + s.vzd;
@@ -190,7 +193,7 @@ __device__ __forceinline__ Lobe eval_lobe(const S& s, const Texel& g) {
     o.om4 = om2 * om2;
     o.F = fmaf(o.om4 * om, 1.f - kF0, kF0);
     o.ia = rcp(fmaf(o.nl, s.omk, s.k));
-    o.Gq = o.nl * o.ia * s.ib4;
+    o.Gq = IB ? o.nl * o.ia * s.ib4 : o.nl * o.ia;
     o.tD = fmaf(o.nh * o.nh, s.a2, sn2 * o.rs2);
     o.tD2 = o.tD * o.tD;
     o.itD2 = rcp(fmaxf(o.tD2, kThresOverPi));
