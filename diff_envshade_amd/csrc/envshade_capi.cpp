@@ -108,12 +108,19 @@ des::Tiling tiling_for(int ntc, int T, bool backward) {
   const size_t staged_fwd = staged + size_t(ntc) * tl.list_cap * sizeof(uint16_t);
   // (the tile's exchanged point fields also use the space from offset 0, first)
   const size_t xch = size_t(des::kPointXchSlots) * 64 * sizeof(float);
-  tl.lds_fwd = std::max({staged_fwd, xch, size_t(ntc) * des::kForwardAcc * 64 * sizeof(float)});
-  tl.lds_bwd = std::max({staged, xch, size_t(ntc) * des::kBackwardAcc * 64 * sizeof(float)});
+  // the map's row and column angles last, apart from all of the above: they
+  // are written with the exchanged point fields and read by the table build
+  const size_t ang = size_t(des::kAngleSlots) * 2 * sizeof(double);
+  const size_t fwd_end =
+      std::max({staged_fwd, xch, size_t(ntc) * des::kForwardAcc * 64 * sizeof(float)});
+  const size_t bwd_end = std::max({staged, xch, size_t(ntc) * des::kBackwardAcc * 64 * sizeof(float)});
+  const size_t fwd_ang = (fwd_end + 15) & ~size_t(15), bwd_ang = (bwd_end + 15) & ~size_t(15);
+  tl.ang_off = int(backward ? bwd_ang : fwd_ang);  // the direction this tiling is for
+  tl.lds_fwd = fwd_ang + ang;
+  tl.lds_bwd = bwd_ang + ang;
   tl.split0 = INT64_MAX;  // set per problem by choose_tiling (single-set launches only)
   tl.slices = 1;
   tl.slice_texels = T;
-  (void)backward;
   return tl;
 }
 

@@ -55,6 +55,7 @@ struct Tiling {
   int dark_off;     // byte offset of the super-chunk's dark words (one uint64 per 64 texels)
   int list_off;     // forward: byte offset of the waves' lit-texel lists (uint16 entries)
   int list_cap;     // forward: entries per wave, ceil(st / ntc) rounded up to 4
+  int ang_off;      // byte offset of the map's row and column angles (kAngleSlots double2)
   size_t lds_fwd;   // dynamic LDS bytes, forward
   size_t lds_bwd;   // dynamic LDS bytes, backward
   // Backward tail slices: tiles [split0, tiles) are shaded as `slices`
@@ -122,6 +123,10 @@ constexpr int kForwardAcc = 6;
 constexpr int kBackwardAcc = 16;
 constexpr int kMaxSuperChunk = 512;  // texels staged in LDS at once (48 B each, 24 KB)
 constexpr int kTexelBytes = 48;
+// Row and column angles ({sin, cos} in FP64) a tile keeps in LDS for the texel
+// table when the map has at most this many rows + columns; larger maps compute
+// them per texel.
+constexpr int kAngleSlots = 128;
 constexpr int kPointXchSlots = 25;  // per-point fields a tile exchanges through LDS
 
 // Main shading kernel, then (tail slices) the combine; ev_start / ev_stop, when

@@ -48,14 +48,16 @@ struct TexelGeo {
   double dw;
 };
 
-__device__ __forceinline__ TexelGeo texel_geometry(int t, int H, int W, const double* M,
-                                                   double kdw) {
-  const int r = t / W;
-  const int x = t - r * W;
-  const int iy = H - 1 - r;
-  double sth, cth, sph, cph;
-  sincospi(((double)iy + 0.5) / (double)H - 0.5, &sth, &cth);
-  sincospi((2.0 * (double)x + 1.0) / (double)W, &sph, &cph);
+// sin and cos of theta for flipped row iy, and of phi for column x
+__device__ __forceinline__ void row_angle(int iy, int H, double& s, double& c) {
+  sincospi(((double)iy + 0.5) / (double)H - 0.5, &s, &c);
+}
+__device__ __forceinline__ void col_angle(int x, int W, double& s, double& c) {
+  sincospi((2.0 * (double)x + 1.0) / (double)W, &s, &c);
+}
+
+__device__ __forceinline__ TexelGeo texel_from_angles(double sth, double cth, double sph,
+                                                      double cph, const double* M, double kdw) {
   const double dx = cth * cph, dy = sth, dz = cth * sph;
   TexelGeo g;
   g.lx = M[0] * dx + M[1] * dy + M[2] * dz;
@@ -63,6 +65,27 @@ __device__ __forceinline__ TexelGeo texel_geometry(int t, int H, int W, const do
   g.lz = M[6] * dx + M[7] * dy + M[8] * dz;
   g.dw = cth * kdw;
   return g;
+}
+
+__device__ __forceinline__ TexelGeo texel_geometry(int t, int H, int W, const double* M,
+                                                   double kdw) {
+  const int r = t / W;
+  const int x = t - r * W;
+  double sth, cth, sph, cph;
+  row_angle(H - 1 - r, H, sth, cth);
+  col_angle(x, W, sph, cph);
+  return texel_from_angles(sth, cth, sph, cph, M, kdw);
+}
+
+// The same from a table of the map's row angles (ang[iy], iy < H) and column
+// angles (ang[H + x]) as {sin, cos}: the same values, computed once per map
+// instead of twice per texel.
+__device__ __forceinline__ TexelGeo texel_geometry(int t, int H, int W, const double2* ang,
+                                                   const double* M, double kdw) {
+  const int r = t / W;
+  const int x = t - r * W;
+  const double2 a = ang[H - 1 - r], b = ang[H + x];
+  return texel_from_angles(a.x, a.y, b.x, b.y, M, kdw);
 }
 
 __device__ __forceinline__ Texel make_texel(const TexelGeo& g, float E0, float E1, float E2) {
