@@ -243,6 +243,7 @@ struct PointGeom {
   float nv, k, omk, ib4, a2, a2pi, c4;
   float n2c;  // 1 - |n|^2 (formed in FP64), for eval_weights_f32
   double nvd, v2d;  // n.v and |v|^2 in FP64, for fwd_pair
+  double a2m1d;     // alpha^2 - 1 in FP64, for fwd_pair
   float cD;         // ib4 * a2pi: the per-point factor of F G D / 4, for fwd_pair
 };
 
@@ -265,6 +266,7 @@ __device__ __forceinline__ PointGeom point_geom(float nx, float ny, float nz, fl
   c.n2c = (float)(1.0 - (c.nxd * c.nxd + c.nyd * c.nyd + c.nzd * c.nzd));
   c.nvd = fma(c.nzd, c.vzd, fma(c.nyd, c.vyd, c.nxd * c.vxd));
   c.v2d = fma(c.vzd, c.vzd, fma(c.vyd, c.vyd, c.vxd * c.vxd));
+  c.a2m1d = (double)c.a2 - 1.0;
   c.cD = c.ib4 * c.a2pi;
   return c;
 }
@@ -279,9 +281,9 @@ __device__ __forceinline__ PointGeom point_geom(float nx, float ny, float nz, fl
 // Every dot product is formed in FP64 from the table's FP64 direction l:
 //   l.n, l.v;  l.hb = |l|^2 + l.v;  |hb|^2 = l.hb + (l.v + |v|^2);  n.hb = l.n + n.v,
 // so the NDF denominator
-//   tD = (n.h)^2 alpha^2 + (1 - (n.h)^2) = ((n.hb)^2 alpha^2 + (|hb|^2 - (n.hb)^2)) / |hb|^2
-// takes |hb|^2 - (n.hb)^2 from FP64 (no cancellation at the peak of sharp
-// lobes), and l.h = l.hb / |hb| gives the Fresnel term.  The two
+//   tD = (n.h)^2 alpha^2 + (1 - (n.h)^2) = (|hb|^2 + (alpha^2 - 1) (n.hb)^2) / |hb|^2
+// takes its numerator from FP64, rounded once (no cancellation at the peak of
+// sharp lobes), and l.h = l.hb / |hb| gives the Fresnel term.  The two
 // reciprocals of G1(l) and D share one: F G D / 4 = F x (ib/4) (alpha^2/pi) /
 // (max(tD^2, thres/pi) (max(l.n,0)(1-k) + k)).  Same terms as eval_lobe.
 template <bool SPEC, bool MASK>
@@ -296,15 +298,14 @@ __device__ __forceinline__ void fwd_pair(const PointGeom& s, const FwdTexel& g, 
   const double lhb = lv + g.l2;
   const double hb2 = lhb + (lv + s.v2d);
   const double nhb = nl + s.nvd;
-  const float sn2 = (float)fma(-nhb, nhb, hb2);  // |hb|^2 (1 - (n.h)^2)
+  const float tdn = (float)fma(s.a2m1d, nhb * nhb, hb2);  // |hb|^2 tD
   const float hb2f = (float)hb2;
-  const float nhbf = (float)nhb;
   // floor keeps the terms finite when v = -l exactly; such pairs are masked below
   const float rs = rsq(fmaxf(hb2f, 1e-20f));
   const float om = 1.f - sat((float)lhb * rs);
   const float om2 = om * om;
   const float F = fmaf(om2 * om2 * om, 1.f - kF0, kF0);
-  const float tD = fmaf(nhbf * nhbf, s.a2, sn2) * (rs * rs);
+  const float tD = tdn * (rs * rs);
   const float den = fmaxf(tD * tD, kThresOverPi) * fmaf(nlf, s.omk, s.k);
   const float tmp = ((F * x) * s.cD) * rcp(den);
   // the reference skips the lobe unless tmp > 0; |l+v| = 0 gives NaN there
