@@ -1270,11 +1270,24 @@ __device__ __forceinline__ void reduce_cols_item(const float* __restrict__ part,
 // point the slices' sums in slice order, then the point's gradients.  They
 // come first so that their loads overlap the column sums of the other blocks,
 // which sum one block of env-map-gradient columns each.
+// Column blocks 2m and 2m + 1 (16 columns, 64 bytes of each partials row) share
+// the rows' 128-byte lines; workgroup ids are dealt to the 8 XCDs in turn, so
+// item i of each 16 and item i + 8, which run on one XCD, take the two halves
+// of one line and its L2 fetches the line once.  A last partial group of 16
+// keeps the identity.
+__device__ __forceinline__ int line_paired_block(int i, int n) {
+  const int g = i & ~15;
+  if (g + 16 > n) return i;
+  return g + 2 * (i & 7) + ((i >> 3) & 1);
+}
+
 template <int V, bool DIFF, bool SPEC>
 __global__ __launch_bounds__(1024) void envgrad_reduce_kernel(Problem pr, Tiling tl, int ncomb) {
   const int64_t nrows = (pr.P + 63) / 64;
   if ((int)blockIdx.x >= ncomb) {
-    reduce_cols_item<kReduceCols>(pr.part, pr.gE, nrows, pr.H, pr.W, (int)blockIdx.x - ncomb);
+    const int ncb = (int)gridDim.x - ncomb;
+    reduce_cols_item<kReduceCols>(pr.part, pr.gE, nrows, pr.H, pr.W,
+                                  line_paired_block((int)blockIdx.x - ncomb, ncb));
     return;
   }
   if (threadIdx.x >= 64) return;
